@@ -1,0 +1,110 @@
+/*
+ * exo_oracle.h -- CPU FP64 restatement of ExoCubed's cubed-sphere hydro update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle and the
+ * reported CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product (exocubed_amd/) never links it.
+ *
+ * It follows the reference loop by loop, on Athena's per-MeshBlock layout
+ * (NVAR, nc3, nc2, nc1) with x1 (radial) fastest, one MeshBlock per panel
+ * (mesh level 2).  Each function cites the reference file:line it restates.
+ *
+ * Parity status per function (see also the header of exo_oracle.c):
+ *   pinned by golden vectors : or_weno5 / or_weno3 (tests/test_weno.cpp:11-29)
+ *   pinned by closed forms   : geometry (areas/volumes), vector transforms,
+ *                              ghost remap of constant / smooth fields,
+ *                              block-Thomas vs dense solve, Riemann consistency
+ *   parity unpinned          : Athena-side semantics not in the reference tree
+ *                              (AddFluxDivergence, SSPRK3 weight order,
+ *                              NewBlockTimeStep, reflecting BC fill) -- restated
+ *                              from upstream Athena++ conventions.
+ */
+#ifndef EXO_ORACLE_H_
+#define EXO_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_EOS_SHALLOW_YZ = 0, OR_EOS_IDEAL = 1 };
+enum { OR_RS_ROE_SW = 0, OR_RS_LMARS_SW = 1, OR_RS_HLLC = 2, OR_RS_LMARS = 3 };
+enum { OR_COR_NONE = 0, OR_COR_FORCING2 = 1, OR_COR_3D = 2 };
+
+typedef struct or_config {
+  int nx;             /* cells per panel edge (N), even */
+  int nx1;            /* radial cells */
+  int eos;            /* OR_EOS_* */
+  int rsolver;        /* OR_RS_* */
+  int implicit_flag;  /* 0, 1 (3x3 partial: unsupported), 9 (5x5 full, x1) */
+  int ntracer;        /* passive scalars */
+  int coriolis;       /* OR_COR_* */
+  int cfl_x1;         /* 1: x1 widths enter the CFL minimum */
+  double x1min, x1max;
+  double gamma;
+  double grav;        /* grav_acc1 (negative = downward); 0 for shallow water */
+  double omega;       /* planetary rotation rate for the Coriolis source */
+  double cfl;
+} or_config;
+
+typedef struct or_solver or_solver;
+
+/* scalar kernels (tests/test_weno.cpp) */
+double or_weno5(double phim2, double phim1, double phi, double phip1, double phip2);
+double or_weno3(double phim1, double phi, double phip1);
+
+/* deterministic transcendental functions shared in algorithm with the device */
+double or_exp(double x);
+double or_log(double x);
+double or_pow(double x, double y);
+
+or_solver *or_create(const or_config *cfg);
+void or_destroy(or_solver *s);
+int or_nvar(const or_solver *s);
+int or_ncells(const or_solver *s, int dim); /* dim 1,2,3 -> nc1,nc2,nc3 */
+
+/* host layout: 6 consecutive MeshBlock arrays, panel p (blockID p+1) first
+ * index: [p][n][k][j][i] with (nc3, nc2, nc1) = (N+6, N+6, nx1+6). */
+void or_set_cons(or_solver *s, const double *u);
+void or_get_cons(const or_solver *s, double *u);
+void or_get_prim(const or_solver *s, double *w);
+void or_set_scalars(or_solver *s, const double *sc); /* conserved s = rho*r */
+void or_get_scalars(const or_solver *s, double *sc);
+
+/* geometry tables, [p][k][j] of nc3*nc2 (cell) values; which: 0 cos, 1 sin,
+ * 2 cos_face2, 3 sin_face2, 4 cos_face3, 5 sin_face3, 6 dx2f_ang, 7 dx3f_ang,
+ * 8 dx2f_ang_face3, 9 dx3f_ang_face2 */
+void or_get_geom2d(const or_solver *s, int which, double *out);
+/* 1-D coordinates: which 0 x1f(nc1+1) 1 x1v(nc1) 2 x2f(nc2+1) 3 x2v 4 x3f 5 x3v
+ * for panel p */
+void or_get_coord(const or_solver *s, int p, int which, double *out);
+/* face areas / volume of one cell of panel p */
+double or_cell_volume(const or_solver *s, int p, int k, int j, int i);
+double or_face_area(const or_solver *s, int p, int dir, int k, int j, int i);
+/* lat/lon of a cell center (radians) */
+void or_latlon(const or_solver *s, int p, int k, int j, double *lat, double *lon);
+
+/* pipeline pieces */
+void or_fill_ghosts(or_solver *s);          /* A7 panel remap on u (and s) */
+void or_cons_to_prim(or_solver *s);         /* A12 + reflecting x1 BC */
+void or_calc_fluxes(or_solver *s);          /* A1+A2+A3+A6+A9 from w */
+void or_get_flux(const or_solver *s, int dir, double *out); /* [p][n][..] */
+double or_new_dt(or_solver *s);             /* A14 */
+void or_stage(or_solver *s, int stage, double dt); /* one SSPRK3 stage */
+double or_step(or_solver *s, int nsteps, double dt_fixed); /* returns last dt */
+void or_set_threads(int n);
+
+/* single-column / single-row shims used by the per-function parity tests */
+void or_implicit_column(const or_config *cfg, int nx1, const double *w_col,
+                        /* (5, nx1+6) prim of one column incl. x1 ghosts */
+                        double *du_col, /* (5, nx1+6) in/out, interior used */
+                        const double *area1, /* nx1+1 face areas */
+                        const double *vol,   /* nx1 volumes */
+                        double dt);
+void or_riemann_face(const or_config *cfg, int ivx, const double *wl,
+                     const double *wr, double *flx, double cth_face,
+                     double cos_face, double sin_face, double cos_cell);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EXO_ORACLE_H_ */
