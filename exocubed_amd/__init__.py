@@ -1,0 +1,243 @@
+"""MI355X-native cubed-sphere finite-volume hydro update (ExoCubed's hot path).
+
+The compute path is the HIP library ``libexocubed.so`` built from
+``exocubed_amd/csrc`` for gfx950; this module only loads it and wraps its C ABI
+(``include/exocubed.h``).  There is no CPU fallback: if the library is missing
+the import of :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libexocubed.so")
+
+EOS_SHALLOW_YZ, EOS_IDEAL = 0, 1
+RS_ROE_SW, RS_LMARS_SW, RS_HLLC, RS_LMARS = 0, 1, 2, 3
+COR_NONE, COR_FORCING2, COR_3D = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class Config(C.Structure):
+    """Mirror of ``exo_config`` (include/exocubed.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int), ("nx1", C.c_int), ("eos", C.c_int), ("rsolver", C.c_int),
+        ("implicit_flag", C.c_int), ("ntracer", C.c_int), ("coriolis", C.c_int),
+        ("cfl_x1", C.c_int), ("x1min", C.c_double), ("x1max", C.c_double),
+        ("gamma", C.c_double), ("grav", C.c_double), ("omega", C.c_double),
+        ("cfl", C.c_double), ("precision", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def make_config(**kw) -> Config:
+    base = dict(nx=8, nx1=3, eos=EOS_SHALLOW_YZ, rsolver=RS_ROE_SW, implicit_flag=0,
+                ntracer=0, coriolis=COR_NONE, cfl_x1=0, x1min=6371000.0, x1max=6371001.0,
+                gamma=1.4, grav=0.0, omega=7.292e-5, cfl=0.4, precision=PREC_FP64)
+    base.update(kw)
+    return Config(**base)
+
+
+_lib = None
+_DP = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load the gfx950 library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    sig = {
+        "exo_create": (C.c_int, [C.POINTER(Config), C.POINTER(H)]),
+        "exo_destroy": (C.c_int, [H]),
+        "exo_last_error": (C.c_char_p, [H]),
+        "exo_nvar": (C.c_int, [H]),
+        "exo_state_doubles": (C.c_size_t, [H]),
+        "exo_upload_state": (C.c_int, [H, C.c_void_p, C.c_void_p]),
+        "exo_download_state": (C.c_int, [H, C.c_void_p, C.c_void_p]),
+        "exo_download_prim": (C.c_int, [H, _DP]),
+        "exo_calc_fluxes": (C.c_int, [H]),
+        "exo_download_flux": (C.c_int, [H, C.c_int, _DP]),
+        "exo_add_flux_divergence": (C.c_int, [H, C.c_double]),
+        "exo_fill_ghosts": (C.c_int, [H]),
+        "exo_implicit_correct": (C.c_int, [H, C.c_double]),
+        "exo_new_dt": (C.c_int, [H, C.POINTER(C.c_double)]),
+        "exo_stage": (C.c_int, [H, C.c_int, C.c_double]),
+        "exo_step": (C.c_int, [H, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+        "exo_sync": (C.c_int, [H]),
+        "exo_profile_begin": (C.c_int, [H]),
+        "exo_profile_end": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int]),
+        "exo_init_state": (C.c_int, [H, C.c_int, C.c_uint64, C.c_void_p, C.c_int, _DP,
+                                     C.c_void_p]),
+        "exo_riemann_face": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, _DP, _DP, _DP]),
+        "exo_weno5_batch": (C.c_int, [H, _DP, _DP, C.c_int]),
+        "exo_math_batch": (C.c_int, [H, C.c_int, _DP, C.c_double, _DP, C.c_int]),
+        "exo_init_state_cfg": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint64, C.c_void_p,
+                                         C.c_int, _DP, C.c_void_p]),
+        "exo_describe": (C.c_int, [C.POINTER(Config), C.c_int, C.c_void_p, C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def init_state(cfg: Config, kind: int, seed: int, params=()):
+    """Seeded synthetic initial state (host only): conserved u and scalars s."""
+    L = lib()
+    nv = 4 if cfg.eos == EOS_SHALLOW_YZ else 5
+    nc1, nc2 = cfg.nx1 + 6, cfg.nx + 6
+    u = np.zeros((6, nv, nc2, nc2, nc1))
+    s = np.zeros((6, cfg.ntracer, nc2, nc2, nc1)) if cfg.ntracer else None
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    rc = L.exo_init_state_cfg(C.byref(cfg), kind, seed,
+                              p.ctypes.data_as(C.c_void_p) if len(p) else None, len(p), u,
+                              None if s is None else s.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise RuntimeError(f"exo_init_state_cfg failed ({rc})")
+    return u, s
+
+
+def describe(cfg: Config, which: int) -> np.ndarray:
+    """Set-up tables of an instance, computed on the host (see exo_describe)."""
+    L = lib()
+    n = C.c_size_t()
+    if L.exo_describe(C.byref(cfg), which, None, 0, C.byref(n)) != 0:
+        raise RuntimeError("exo_describe failed")
+    if which == 0:
+        dt = GHOST_DTYPE
+    elif which == 1:
+        dt = EDGE_DTYPE
+    else:
+        dt = np.float64
+    out = np.zeros(n.value, dtype=dt)
+    if L.exo_describe(C.byref(cfg), which, out.ctypes.data_as(C.c_void_p), out.nbytes,
+                      C.byref(n)) != 0:
+        raise RuntimeError("exo_describe failed")
+    return out
+
+
+GHOST_DTYPE = np.dtype([(f, np.int32) for f in
+                        ("tgt_panel", "tgt_k", "tgt_j", "src_panel", "along_x2", "src_row",
+                         "src_a0", "src_block", "tgt_block", "pad")] +
+                       [(f, np.float64) for f in
+                        ("A", "B", "D", "src_cy", "src_cz", "tgt_cy", "tgt_cz", "sth_s",
+                         "cth_s", "sth_t", "cth_t")])
+EDGE_DTYPE = np.dtype([(f, np.int32) for f in
+                       ("src_panel", "src_dir", "src_k", "src_j", "tgt_panel", "tgt_dir",
+                        "tgt_a", "src_block", "tgt_block", "pad")] +
+                      [(f, np.float64) for f in ("x2s", "x3s", "x2d", "x3d")])
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class ExoError(RuntimeError):
+    pass
+
+
+class Solver:
+    """One cubed-sphere instance resident on the GPU (an ``exo_handle``)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.exo_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise ExoError(f"exo_create failed ({rc})")
+        self.h = h
+        self.nv = 4 if cfg.eos == EOS_SHALLOW_YZ else 5
+        self.nc1, self.nc2 = cfg.nx1 + 6, cfg.nx + 6
+        self.shape = (6, self.nv, self.nc2, self.nc2, self.nc1)
+        self.sshape = (6, cfg.ntracer, self.nc2, self.nc2, self.nc1)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.exo_last_error(self.h).decode()
+            raise ExoError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.exo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_state(self, kind, seed, params=()):
+        return init_state(self.cfg, kind, seed, params)
+
+    def upload(self, u, s=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        s = None if s is None else np.ascontiguousarray(s, dtype=np.float64)
+        self._chk(self.L.exo_upload_state(self.h, _ptr(u), _ptr(s)), "exo_upload_state")
+
+    def download(self):
+        u = np.zeros(self.shape)
+        s = np.zeros(self.sshape) if self.cfg.ntracer else None
+        self._chk(self.L.exo_download_state(self.h, _ptr(u), _ptr(s)), "exo_download_state")
+        return u, s
+
+    def prim(self):
+        w = np.zeros(self.shape)
+        self._chk(self.L.exo_download_prim(self.h, w), "exo_download_prim")
+        return w
+
+    def calc_fluxes(self):
+        self._chk(self.L.exo_calc_fluxes(self.h), "exo_calc_fluxes")
+
+    def flux(self, d):
+        n1 = self.nc1 + (d == 0)
+        n2 = self.nc2 + (d == 1)
+        n3 = self.nc2 + (d == 2)
+        out = np.zeros((6, self.nv, n3, n2, n1))
+        self._chk(self.L.exo_download_flux(self.h, d, out), "exo_download_flux")
+        return out
+
+    def step(self, nsteps=1, dt=0.0):
+        d = C.c_double()
+        self._chk(self.L.exo_step(self.h, nsteps, dt, C.byref(d)), "exo_step")
+        return d.value
+
+    def stage(self, stage, dt):
+        self._chk(self.L.exo_stage(self.h, stage, dt), "exo_stage")
+
+    def new_dt(self):
+        d = C.c_double()
+        self._chk(self.L.exo_new_dt(self.h, C.byref(d)), "exo_new_dt")
+        return d.value
+
+    KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "update",
+               "implicit", "dt")
+
+    def profile_begin(self):
+        self._chk(self.L.exo_profile_begin(self.h), "exo_profile_begin")
+
+    def profile_end(self):
+        ms = np.zeros(len(self.KERNELS))
+        cnt = np.zeros(len(self.KERNELS), dtype=np.int32)
+        self._chk(self.L.exo_profile_end(self.h, ms.ctypes.data_as(C.c_void_p),
+                                         cnt.ctypes.data_as(C.c_void_p), len(ms)),
+                  "exo_profile_end")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNELS)}
+
+    def sync(self):
+        self._chk(self.L.exo_sync(self.h), "exo_sync")

@@ -1,0 +1,1392 @@
+// exo_kernels.hip -- gfx950 kernels of the cubed-sphere finite-volume update
+// and the C ABI of include/exocubed.h.
+//
+// Device layout of every cell-centred array of one instance:
+//   [var][panel][k][i][j], j (x2) fastest, row pitch jp >= nc2 + 1
+// so that wavefront lanes walk j: x1 column sweeps and x3 sweeps stream
+// coalesced rows, x2 sweeps read neighbouring lanes' rows.  Face fluxes use
+// the same layout with one extra k plane and one extra i row.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/exocubed.h"
+#include "exo_device.hpp"
+#include "exo_setup.hpp"
+
+using namespace exo;
+
+#define HIPCHK(x)                                                              \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) throw std::string("HIP: ") + hipGetErrorString(e_) + \
+                                " at " #x;                                     \
+  } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// kernel-side view of one instance
+struct Grid {
+  int N, nx1, nc1, nc2, nc3, jp, nv, nt;
+  int is, ie, js, je, ks, ke;
+  int eos, rs, coriolis, implicit, cfl_x1;
+  double gamma, grav, cfl;
+  size_t plane, pstride, vstride;      // cell arrays
+  size_t fplane, fpstride, fvstride;   // face arrays
+  // geometry
+  const double *x1f, *x1v, *dx1f;
+  const double *tab;                   // [T_NTAB][6][nc3][nc2]
+  size_t t2stride;                     // 6*nc3*nc2
+  __host__ __device__ size_t c(int v, int p, int k, int i, int j) const {
+    return v * vstride + p * pstride + k * plane + (size_t)i * jp + j;
+  }
+  __host__ __device__ size_t f(int v, int p, int k, int i, int j) const {
+    return v * fvstride + p * fpstride + k * fplane + (size_t)i * jp + j;
+  }
+  __device__ double T(int t, int p, int k, int j) const {
+    return tab[t * t2stride + ((size_t)p * nc3 + k) * nc2 + j];
+  }
+  __device__ double area1(int p, int k, int j, int i) const {
+    return T(T_DX2A, p, k, j) * T(T_DX3A, p, k, j) * x1f[i] * x1f[i] * T(T_SINC, p, k, j);
+  }
+  __device__ double area2(int p, int k, int j, int i) const {
+    return x1v[i] * T(T_DX3A_F2, p, k, j) * dx1f[i];
+  }
+  __device__ double area3(int p, int k, int j, int i) const {
+    return x1v[i] * T(T_DX2A_F3, p, k, j) * dx1f[i];
+  }
+  __device__ double volume(int p, int k, int j, int i) const {
+    return dx1f[i] * T(T_DX2A, p, k, j) * T(T_DX3A, p, k, j) * x1v[i] * x1v[i] *
+           T(T_SINC, p, k, j);
+  }
+};
+
+struct State {
+  double *u, *w, *u1, *du;       // nv cell arrays
+  double *s, *r, *s1;            // nt cell arrays
+  double *flx[3], *sflx[3];      // face arrays
+  double *edge;                  // [nv][6][4][N][nx1] ghost-side face states
+  double *coef;                  // implicit column coefficients, 30 per cell
+  unsigned long long *dtbits;    // CFL minimum as ordered bits
+  double *dtv;                   // [0] dt of the step, [1] previous dt
+};
+
+constexpr int BLK = 256;
+
+// ---------------------------------------------------------------------------
+// EquationOfState::ConservedToPrimitive (shallow_yz_hydro.cpp:42-67,
+// ideal_moist_hydro.cpp:77-159) on interior and face-ghost columns, then the
+// reflecting x1 boundary (v1 mirrored).  One thread per (panel, k, j) column.
+__global__ void k_prim(Grid g, State st, double dfloor, double sfloor) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = blockIdx.y, p = blockIdx.z;
+  if (j >= g.nc2) return;
+  bool kin = k >= g.ks && k <= g.ke, jin = j >= g.js && j <= g.je;
+  if (!kin && !jin) return;  // corner ghosts are never read
+  double cth = g.T(T_COSC, p, k, j);
+  for (int i = g.is; i <= g.ie; ++i) {
+    if (g.eos == EOS_SW) {
+      double ud = st.u[g.c(IDN, p, k, i, j)];
+      ud = (ud > dfloor) ? ud : dfloor;
+      st.u[g.c(IDN, p, k, i, j)] = ud;
+      double v = st.u[g.c(IVY, p, k, i, j)], ww = st.u[g.c(IVZ, p, k, i, j)];
+      double di = 1. / ud;
+      double sth2 = 1. - cth * cth;
+      double wy = v / sth2 - ww * cth / sth2;
+      double wz = -v * cth / sth2 + ww / sth2;
+      st.w[g.c(IDN, p, k, i, j)] = ud;
+      st.w[g.c(IVX, p, k, i, j)] = 0.;
+      st.w[g.c(IVY, p, k, i, j)] = wy * di;
+      st.w[g.c(IVZ, p, k, i, j)] = wz * di;
+    } else {
+      double gm1 = g.gamma - 1.0;
+      double u_d = st.u[g.c(IDN, p, k, i, j)], u_m1 = st.u[g.c(IVX, p, k, i, j)];
+      double u_m2 = st.u[g.c(IVY, p, k, i, j)], u_m3 = st.u[g.c(IVZ, p, k, i, j)];
+      double u_e = st.u[g.c(IEN, p, k, i, j)];
+      double density = 0.;
+      density += u_d;
+      double di = 1. / density;
+      double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
+      double sth2 = 1. - cth * cth;
+      double v = w_vy, ww = w_vz;
+      w_vy = v / sth2 - ww * cth / sth2;
+      w_vz = -v * cth / sth2 + ww / sth2;
+      double KE = 0.5 * (u_m1 * w_vx + u_m2 * w_vy + u_m3 * w_vz);
+      st.w[g.c(IDN, p, k, i, j)] = density;
+      st.w[g.c(IVX, p, k, i, j)] = w_vx;
+      st.w[g.c(IVY, p, k, i, j)] = w_vy;
+      st.w[g.c(IVZ, p, k, i, j)] = w_vz;
+      st.w[g.c(IPR, p, k, i, j)] = gm1 * (u_e - KE) * 1. / 1.;
+    }
+    for (int n = 0; n < g.nt; ++n) {
+      double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / st.u[g.c(IDN, p, k, i, j)]);
+      st.r[g.c(n, p, k, i, j)] = rr < sfloor ? sfloor : rr;
+    }
+  }
+  for (int n = 0; n < g.nv; ++n)
+    for (int q = 1; q <= NG; ++q) {
+      double lo = st.w[g.c(n, p, k, g.is + q - 1, j)];
+      double hi = st.w[g.c(n, p, k, g.ie - q + 1, j)];
+      st.w[g.c(n, p, k, g.is - q, j)] = n == IVX ? -lo : lo;
+      st.w[g.c(n, p, k, g.ie + q, j)] = n == IVX ? -hi : hi;
+    }
+  for (int n = 0; n < g.nt; ++n)
+    for (int q = 1; q <= NG; ++q) {
+      st.r[g.c(n, p, k, g.is - q, j)] = st.r[g.c(n, p, k, g.is + q - 1, j)];
+      st.r[g.c(n, p, k, g.ie + q, j)] = st.r[g.c(n, p, k, g.ie - q + 1, j)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Panel-edge ghost remap (CubedSphereUtility::PackData with InteprolateX2/X3,
+// cubed_sphere_utility.cpp:92-617): one thread per (ghost cell, i); the
+// descriptor carries the interpolation weights and vector-map points.
+__global__ void k_ghost(Grid g, const GhostDesc *__restrict__ gd, int ndesc, double *arr,
+                        int nvar, int vectors) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ni = g.nx1;
+  if (t >= ndesc * ni) return;
+  int di = t / ni, i = g.is + t % ni;
+  GhostDesc d = gd[di];
+  int sp = d.src_panel;
+  size_t o0, o1;
+  if (d.along_x2) {
+    o0 = g.c(0, sp, d.src_row, i, d.src_a0);
+    o1 = g.c(0, sp, d.src_row, i, d.src_a0 + 1);
+  } else {
+    o0 = g.c(0, sp, d.src_a0, i, d.src_row);
+    o1 = g.c(0, sp, d.src_a0 + 1, i, d.src_row);
+  }
+  size_t ot = g.c(0, d.tgt_panel, d.tgt_k, i, d.tgt_j);
+  for (int n = 0; n < nvar; ++n) {
+    if (vectors && (n == IVY || n == IVZ)) continue;
+    double v1 = arr[n * g.vstride + o0], v2 = arr[n * g.vstride + o1];
+    arr[n * g.vstride + ot] = (d.A * v1 + d.B * v2) / d.D;
+  }
+  if (vectors) {
+    double v1y = arr[IVY * g.vstride + o0], v1z = arr[IVZ * g.vstride + o0];
+    double v2y = arr[IVY * g.vstride + o1], v2z = arr[IVZ * g.vstride + o1];
+    double vy = (d.A * v1y + d.B * v2y) / d.D;
+    double vz = (d.A * v1z + d.B * v2z) / d.D;
+    double sth = d.sth_s, cth = d.cth_s;
+    double vyt = vy, vzt = vz;
+    vy = vyt / sth / sth - vzt * cth / sth / sth;
+    vz = vzt / sth / sth - vyt * cth / sth / sth;
+    double U, V;
+    vec_rll_from_abp(d.src_cy, d.src_cz, d.src_block, vy, vz, &U, &V);
+    vec_abp_from_rll(d.tgt_cy, d.tgt_cz, d.tgt_block, U, V, &vy, &vz);
+    sth = d.sth_t;
+    cth = d.cth_t;
+    vyt = vy;
+    vzt = vz;
+    vy = vyt + vzt * cth;
+    vz = vzt + vyt * cth;
+    arr[IVY * g.vstride + ot] = vy;
+    arr[IVZ * g.vstride + ot] = vz;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x1 (radial) fluxes, one thread per interior column: the hydrostatic
+// decomposition of Decomposition::ChangeToPerturbation / RestoreFromPerturbation
+// (change_to_perturbation.cpp:39-190) around Weno5X1, then the x1 Riemann
+// solve (patches/19.calculate_fluxes.patch:77-106).  The face pressure psf
+// lives in LDS, [face][lane].
+__global__ void __launch_bounds__(64) k_flux_x1(Grid g, State st) {
+  extern __shared__ double lds[];
+  int lane = threadIdx.x;
+  int col = blockIdx.x * 64 + lane;
+  int p = blockIdx.y;
+  int N = g.N;
+  bool act = col < N * N;
+  int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
+  double *psf = lds;  // (nc1 + 1) x 64
+#define PSF(ii) psf[(ii) * 64 + lane]
+  int is = g.is, ie = g.ie, nv = g.nv;
+  bool decomp = g.grav != 0.;
+  double grav = -g.grav;
+  double e1 = 1., ig = 1.;
+  if (act && decomp) {
+    double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
+    double RdTv = pie / die;
+    PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * RdTv));
+    for (int i = ie; i >= is; --i)
+      PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
+    for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
+    for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
+    double ent0 = g.gamma;
+    double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
+    ig = 1. / ent0;
+    e1 = dexp(-ent1 / ent0);
+  }
+  if (!act) return;
+  // window of 6 cells (f-3 .. f+2) for face f, perturbed IDN / IPR
+  double q[6][5];
+  auto load = [&](int slot, int i) {
+    for (int n = 0; n < nv; ++n) q[slot][n] = st.w[g.c(n, p, k, i, j)];
+    if (decomp) {
+      double a = PSF(i), b = PSF(i + 1), psv;
+      if (fabs(a - b) < 1.E-6) psv = (a + b) / 2.;
+      else psv = (a - b) / dlog(a / b);
+      double dsv = dpow(psv, ig) * e1;
+      q[slot][IPR] -= psv;
+      q[slot][IDN] -= dsv;
+    }
+  };
+#pragma unroll
+  for (int s = 0; s < 6; ++s) load(s, is - 3 + s);
+  double cth = g.T(T_CTH1, p, k, j), cosc = g.T(T_COSC, p, k, j);
+  for (int fi = is; fi <= ie + 1; ++fi) {
+    double wl[5], wr[5], fl[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      if (n < nv) {
+        wl[n] = weno5(q[4][n], q[3][n], q[2][n], q[1][n], q[0][n]);
+        wr[n] = weno5(q[1][n], q[2][n], q[3][n], q[4][n], q[5][n]);
+      }
+    }
+    if (decomp) {
+      double pf = PSF(fi);
+      wr[IPR] += pf;
+      if (wr[IPR] < 0.) wr[IPR] = pf;
+      wl[IPR] += pf;
+      if (wl[IPR] < 0.) wl[IPR] = pf;
+      double dsf = dpow(pf, ig) * e1;
+      wr[IDN] += dsf;
+      if (wr[IDN] < 0.) wr[IDN] = dsf;
+      wl[IDN] += dsf;
+      if (wl[IDN] < 0.) wl[IDN] = dsf;
+    }
+    riemann<double>(g.eos, g.rs, IVX, g.gamma, wl, wr, fl, cth, 0., 1., cosc);
+    for (int n = 0; n < nv; ++n) st.flx[0][g.f(n, p, k, fi, j)] = fl[n];
+    if (fi <= ie) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) q[s][n] = q[s + 1][n];
+      load(5, fi + 3);
+    }
+  }
+  // passive scalars: WENO of r along x1, upwinded by the mass flux
+  for (int n = 0; n < g.nt; ++n)
+    for (int fi = is; fi <= ie + 1; ++fi) {
+      double r0 = st.r[g.c(n, p, k, fi - 3, j)], r1 = st.r[g.c(n, p, k, fi - 2, j)];
+      double r2 = st.r[g.c(n, p, k, fi - 1, j)], r3 = st.r[g.c(n, p, k, fi, j)];
+      double r4 = st.r[g.c(n, p, k, fi + 1, j)], r5 = st.r[g.c(n, p, k, fi + 2, j)];
+      double rl = weno5(r4, r3, r2, r1, r0);
+      double rr = weno5(r1, r2, r3, r4, r5);
+      double m = st.flx[0][g.f(IDN, p, k, fi, j)];
+      st.sflx[0][g.f(n, p, k, fi, j)] = m >= 0.0 ? m * rl : m * rr;
+    }
+#undef PSF
+}
+
+// ---------------------------------------------------------------------------
+// Panel-edge L/R exchange (CubedSphere::sendNeighborBlocks, cubed_sphere.cpp:
+// 326-576): the interior-side WENO state of each edge face, its IVY / IVZ
+// mapped through lat-lon into the neighbour's frame, stored in the
+// neighbour's ghost-side slot [n][panel][side][a][i].
+__global__ void k_edge_states(Grid g, State st, const EdgeDesc *__restrict__ ed, int nedge) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ni = g.nx1;
+  if (t >= nedge * ni) return;
+  int e = t / ni, i = g.is + t % ni;
+  EdgeDesc d = ed[e];
+  int p = d.src_panel, k = d.src_k, j = d.src_j;
+  double q[5];
+  for (int n = 0; n < g.nv; ++n) {
+    double v;
+    if (d.src_dir == 0)  // R state of cell js at face js
+      v = weno5(st.w[g.c(n, p, k, i, j - 2)], st.w[g.c(n, p, k, i, j - 1)],
+                st.w[g.c(n, p, k, i, j)], st.w[g.c(n, p, k, i, j + 1)],
+                st.w[g.c(n, p, k, i, j + 2)]);
+    else if (d.src_dir == 1)  // L state of cell je at face je+1
+      v = weno5(st.w[g.c(n, p, k, i, j + 1)], st.w[g.c(n, p, k, i, j)],
+                st.w[g.c(n, p, k, i, j - 1)], st.w[g.c(n, p, k, i, j - 2)],
+                st.w[g.c(n, p, k, i, j - 3)]);
+    else if (d.src_dir == 2)
+      v = weno5(st.w[g.c(n, p, k - 2, i, j)], st.w[g.c(n, p, k - 1, i, j)],
+                st.w[g.c(n, p, k, i, j)], st.w[g.c(n, p, k + 1, i, j)],
+                st.w[g.c(n, p, k + 2, i, j)]);
+    else
+      v = weno5(st.w[g.c(n, p, k + 1, i, j)], st.w[g.c(n, p, k, i, j)],
+                st.w[g.c(n, p, k - 1, i, j)], st.w[g.c(n, p, k - 2, i, j)],
+                st.w[g.c(n, p, k - 3, i, j)]);
+    q[n] = v;
+  }
+  double U, V, vx, vy;
+  vec_rll_from_abp(d.x2s, d.x3s, d.src_block, q[IVY], q[IVZ], &U, &V);
+  vec_abp_from_rll(d.x2d, d.x3d, d.tgt_block, U, V, &vx, &vy);
+  q[IVY] = vx;
+  q[IVZ] = vy;
+  size_t base = (((size_t)d.tgt_panel * 4 + d.tgt_dir) * g.N + d.tgt_a) * ni + (i - g.is);
+  size_t vs = (size_t)6 * 4 * g.N * ni;
+  for (int n = 0; n < g.nv; ++n) st.edge[n * vs + base] = q[n];
+}
+
+// x2 faces (dir = 1) or x3 faces (dir = 2): WENO5 L/R of the face, the panel
+// edge's ghost side from the exchanged states, and Hydro::RiemannSolver with
+// the cubed-sphere frame maps.  One thread per (panel, k, i, j) face.
+__global__ void k_flux_h(Grid g, State st, int dir) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  int na = N + 1;  // faces along the normal axis
+  size_t total = (size_t)6 * N * ni * na;
+  if (t >= total) return;
+  // lanes walk j in both directions: x2 faces (j fastest over N+1 faces),
+  // x3 faces (j fastest over N cells, face index k slower)
+  int kk, fa, i, p;
+  if (dir == 1) {
+    fa = (int)(t % na);
+    size_t r = t / na;
+    i = g.is + (int)(r % ni);
+    r /= ni;
+    kk = (int)(r % N);
+    p = (int)(r / N);
+  } else {
+    kk = (int)(t % N);
+    size_t r = t / N;
+    i = g.is + (int)(r % ni);
+    r /= ni;
+    fa = (int)(r % na);
+    p = (int)(r / na);
+  }
+  int k, j;
+  if (dir == 1) { k = g.ks + kk; j = g.js + fa; }
+  else { k = g.ks + fa; j = g.js + kk; }
+  double wl[5], wr[5], fl[5];
+  for (int n = 0; n < g.nv; ++n) {
+    double m3, m2, m1, c0, p1, p2;
+    if (dir == 1) {
+      m3 = st.w[g.c(n, p, k, i, j - 3)];
+      m2 = st.w[g.c(n, p, k, i, j - 2)];
+      m1 = st.w[g.c(n, p, k, i, j - 1)];
+      c0 = st.w[g.c(n, p, k, i, j)];
+      p1 = st.w[g.c(n, p, k, i, j + 1)];
+      p2 = st.w[g.c(n, p, k, i, j + 2)];
+    } else {
+      m3 = st.w[g.c(n, p, k - 3, i, j)];
+      m2 = st.w[g.c(n, p, k - 2, i, j)];
+      m1 = st.w[g.c(n, p, k - 1, i, j)];
+      c0 = st.w[g.c(n, p, k, i, j)];
+      p1 = st.w[g.c(n, p, k + 1, i, j)];
+      p2 = st.w[g.c(n, p, k + 2, i, j)];
+    }
+    wl[n] = weno5(p1, c0, m1, m2, m3);
+    wr[n] = weno5(m2, m1, c0, p1, p2);
+  }
+  size_t vs = (size_t)6 * 4 * N * ni;
+  int lo_side = dir == 1 ? 0 : 2;
+  if (fa == 0) {  // ghost side is the L state
+    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
+    for (int n = 0; n < g.nv; ++n) wl[n] = st.edge[n * vs + b];
+  } else if (fa == N) {  // ghost side is the R state
+    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
+    for (int n = 0; n < g.nv; ++n) wr[n] = st.edge[n * vs + b];
+  }
+  double cosf, sinf, cth;
+  if (dir == 1) {
+    cosf = g.T(T_COS2, p, k, j);
+    sinf = g.T(T_SIN2, p, k, j);
+    cth = g.T(T_CTH2, p, k, j);
+  } else {
+    cosf = g.T(T_COS3, p, k, j);
+    sinf = g.T(T_SIN3, p, k, j);
+    cth = g.T(T_CTH3, p, k, j);
+  }
+  riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
+  double *F = st.flx[dir];
+  for (int n = 0; n < g.nv; ++n) F[g.f(n, p, k, i, j)] = fl[n];
+  for (int n = 0; n < g.nt; ++n) {
+    double r[6];
+    for (int s = 0; s < 6; ++s)
+      r[s] = dir == 1 ? st.r[g.c(n, p, k, i, j - 3 + s)] : st.r[g.c(n, p, k - 3 + s, i, j)];
+    double rl = weno5(r[4], r[3], r[2], r[1], r[0]);
+    double rr = weno5(r[1], r[2], r[3], r[4], r[5]);
+    double m = fl[IDN];
+    st.sflx[dir][g.f(n, p, k, i, j)] = m >= 0.0 ? m * rl : m * rr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage update of one interior cell: SSPRK3 weighted average (u1, u),
+// Hydro::AddFluxDivergence, GnomonicEquiangle::AddCoordTermsDivergence
+// (gnomonic_equiangle.cpp:834-879), ConstantAcceleration
+// (patches/25.constant_acc.patch:19-84) and the Coriolis user source
+// (W92.cpp:48-70 / hs94.cpp rotation term).  Shallow water updates u
+// directly; the ideal gas writes the tendency du (ImplicitHydroTasks).
+struct StageW {
+  double delta, g1, g2, beta;
+  int first;
+};
+
+__global__ void k_update(Grid g, State st, StageW sw) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * N * ni * N;
+  if (t >= total) return;
+  int j = g.js + (int)(t % N);
+  size_t r = t / N;
+  int i = g.is + (int)(r % ni);
+  r /= ni;
+  int k = g.ks + (int)(r % N), p = (int)(r / N);
+  double dt = st.dtv[0];
+  double wght = sw.beta * dt;
+  bool tend = g.eos == EOS_IDEAL;
+  double out[5];
+  double a1l = g.area1(p, k, j, i), a1r = g.area1(p, k, j, i + 1);
+  double a2l = g.area2(p, k, j, i), a2r = g.area2(p, k, j + 1, i);
+  double a3l = g.area3(p, k, j, i), a3r = g.area3(p, k + 1, j, i);
+  double vol = g.volume(p, k, j, i);
+  for (int n = 0; n < g.nv; ++n) {
+    size_t c = g.c(n, p, k, i, j);
+    double u = st.u[c], u1 = st.u1[c];
+    if (sw.first) u1 = 0.;
+    if (sw.delta != 0.0) u1 += sw.delta * u;
+    if (sw.first) u = u1;
+    else u = sw.g1 * u + sw.g2 * u1;
+    st.u1[c] = u1;
+    double dflx;
+    if (g.eos == EOS_SW) dflx = 0.;
+    else
+      dflx = a1r * st.flx[0][g.f(n, p, k, i + 1, j)] - a1l * st.flx[0][g.f(n, p, k, i, j)];
+    dflx += a2r * st.flx[1][g.f(n, p, k, i, j + 1)] - a2l * st.flx[1][g.f(n, p, k, i, j)];
+    dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
+    if (tend) {
+      st.u[c] = u;
+      out[n] = 0.;
+      out[n] -= wght * dflx / vol;
+    } else {
+      out[n] = u;
+      out[n] -= wght * dflx / vol;
+    }
+  }
+  // coordinate source terms
+  double cosine = g.T(T_COSC, p, k, j), sinc = g.T(T_SINC, p, k, j);
+  double sine2 = sinc * sinc;
+  double v1 = st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
+  double v3 = st.w[g.c(IVZ, p, k, i, j)];
+  double rho = st.w[g.c(IDN, p, k, i, j)];
+  double radius = g.x1v[i];
+  double v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
+  double pr;
+  if (g.eos == EOS_SW) {
+    pr = 0.5 * rho * rho;
+  } else {
+    pr = st.w[g.c(IPR, p, k, i, j)];
+    double src1 = (2.0 * pr + rho * (v2 * v_2 + v3 * v_3)) / radius;
+    out[IVX] += wght * src1;
+  }
+  // x_ov_rD / y_ov_rC of the constructor (gnomonic_equiangle.cpp:209-218)
+  double xovrD = (a2l * g.T(T_SIN2, p, k, j) - a2r * g.T(T_SIN2, p, k, j + 1)) / vol;
+  double yovrC = (a3l * g.T(T_SIN3, p, k, j) - a3r * g.T(T_SIN3, p, k + 1, j)) / vol;
+  double src2 = -xovrD * (pr + rho * v3 * v3 * sine2) - rho * v1 * v_2 / radius;
+  out[IVY] += wght * src2;
+  double src3 = -yovrC * (pr + rho * v2 * v2 * sine2) - rho * v1 * v_3 / radius;
+  out[IVZ] += wght * src3;
+  // sources
+  if (g.eos == EOS_IDEAL && g.grav != 0.0) {
+    double src = wght * rho * g.grav;
+    out[IVX] += src;
+    out[IEN] += src * v1;
+  }
+  if (g.coriolis != 0) {
+    double X = g.T(T_TANX, p, k, j), Y = g.T(T_TANY, p, k, j), f = g.T(T_CORF, p, k, j);
+    double U, V, acc2, acc3;
+    vec_rll_from_abp(X, Y, p + 1, v2, v3, &U, &V);  // CubedSphere::GetUV
+    U = -U;
+    double llU, llV;
+    if (g.coriolis == 1) {
+      llU = f * V;
+      llV = -f * U;
+    } else {
+      double m2 = rho * U, m3 = rho * V;
+      llU = f * m3;
+      llV = -f * m2;
+    }
+    vec_abp_from_rll(X, Y, p + 1, -llU, llV, &acc2, &acc3);  // GetVyVz
+    double C = sqrt(X * X + 1), D = sqrt(Y * Y + 1);
+    double cth = -X * Y / (C * D);
+    double a2c = acc2 + acc3 * cth, a3c = acc3 + acc2 * cth;
+    if (g.coriolis == 1) {
+      out[IVY] += wght * rho * a2c;
+      out[IVZ] += wght * rho * a3c;
+    } else {
+      out[IVY] += wght * a2c;
+      out[IVZ] += wght * a3c;
+    }
+  }
+  double *dst = tend ? st.du : st.u;
+  for (int n = 0; n < g.nv; ++n) dst[g.c(n, p, k, i, j)] = out[n];
+  // passive scalars: weighted average and flux divergence into s
+  for (int n = 0; n < g.nt; ++n) {
+    size_t c = g.c(n, p, k, i, j);
+    double s = st.s[c], s1 = st.s1[c];
+    if (sw.first) s1 = 0.;
+    if (sw.delta != 0.0) s1 += sw.delta * s;
+    if (sw.first) s = s1;
+    else s = sw.g1 * s + sw.g2 * s1;
+    st.s1[c] = s1;
+    double dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
+    if (g.eos == EOS_SW) dflx = 0.;
+    dflx += a2r * st.sflx[1][g.f(n, p, k, i, j + 1)] - a2l * st.sflx[1][g.f(n, p, k, i, j)];
+    dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
+    s -= wght * dflx / vol;
+    st.s[c] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ImplicitSolver::FullCorrection on an x1 column (full_correction.cpp:32-165,
+// forward_backward.hpp:15-126), one thread per column; the forward sweep keeps
+// the previous block row in registers and spills a[i] and delta[i] to the
+// coefficient array for the backward substitution.  Ends with u += du.
+__global__ void __launch_bounds__(64) k_implicit(Grid g, State st, double beta) {
+  int col = blockIdx.x * 64 + threadIdx.x;
+  int p = blockIdx.y, N = g.N;
+  if (col >= N * N) return;
+  int k = g.ks + col / N, j = g.js + col % N;
+  int is = g.is, ie = g.ie;
+  double dt = beta * st.dtv[0];
+  double gamma = g.gamma, grav = g.grav;
+  double cth = g.T(T_COSC, p, k, j), sth = g.T(T_SINC, p, k, j);
+  auto wproj = [&](int i, double *w) {
+    for (int n = 0; n < 5; ++n) w[n] = st.w[g.c(n, p, k, i, j)];
+    w[IVY] += w[IVZ] * cth;
+    w[IVZ] *= sth;
+  };
+  double gm1c = (gamma - 1.) * 1. / 1.;
+  double gm1 = 0.5 * (gm1c + gm1c);
+  // |A| at face is-1/2: the reference's row is-1 (full_correction.cpp:272-325)
+  // leaves Am = R|L|R^-1 of the Roe state of (w(is-1), w(is))
+  double wm[5], w0[5], wp[5], prim[5];
+  wproj(is - 1, wm);
+  wproj(is, w0);
+  roe_average(prim, gm1, wm, w0);
+  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  M5 Am = abs_a(prim, cs, gm1);
+  // wm = w(i-1), w0 = w(i), wp = w(i+1) at row i
+  M5 Aprev;
+  V5 dprev;
+  size_t cs30 = (size_t)g.c(0, p, k, 0, j);  // column base in the coefficient array
+  for (int i = is; i <= ie; ++i) {
+    wproj(i + 1, wp);
+    roe_average(prim, gm1, w0, wp);
+    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    M5 Ap = abs_a(prim, cs, gm1);
+    M5 Jm = flux_jacobian(gm1c, wm), J0 = flux_jacobian(gm1c, w0), Jp = flux_jacobian(gm1c, wp);
+    double aleft = g.area1(p, k, j, i), aright = g.area1(p, k, j, i + 1);
+    double v = g.volume(p, k, j, i);
+    M5 a, b, c;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double dtq = r == q ? 1. / dt : 0.;
+        double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
+        a.a[r][q] = (Am.a[r][q] * aleft + Ap.a[r][q] * aright + (aright - aleft) * J0.a[r][q]) /
+                        (2. * v) +
+                    dtq - phi;
+        b.a[r][q] = -(Am.a[r][q] + Jm.a[r][q]) * aleft / (2. * v);
+        c.a[r][q] = -(Ap.a[r][q] - Jp.a[r][q]) * aright / (2. * v);
+      }
+    if (i == is)
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a.a[r][q] += b.a[r][q] * (q == IVX ? -1. : 1.);
+    if (i == ie)
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a.a[r][q] += c.a[r][q] * (q == IVX ? -1. : 1.);
+    V5 rhs;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) rhs.a[n] = st.du[g.c(n, p, k, i, j)] / dt;
+    M5 ainv;
+    V5 delta;
+    if (i == is) {
+      ainv = m5_inv(a);
+      delta = m5_mulv(ainv, rhs);
+    } else {
+      M5 ba = m5_mul(b, Aprev);
+      M5 tt;
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) tt.a[r][q] = a.a[r][q] - ba.a[r][q];
+      ainv = m5_inv(tt);
+      V5 bd = m5_mulv(b, dprev);
+      V5 tv;
+#pragma unroll
+      for (int n = 0; n < 5; ++n) tv.a[n] = rhs.a[n] - bd.a[n];
+      delta = m5_mulv(ainv, tv);
+    }
+    Aprev = m5_mul(ainv, c);
+    dprev = delta;
+    double *cf = st.coef;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp] = Aprev.a[r][q];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) cf[(size_t)(25 + n) * g.vstride + cs30 + (size_t)i * g.jp] = delta.a[n];
+    Am = Ap;
+    for (int n = 0; n < 5; ++n) {
+      wm[n] = w0[n];
+      w0[n] = wp[n];
+    }
+  }
+  // backward substitution; delta(ie) is final
+  V5 dnext = dprev;
+  for (int n = 0; n < 5; ++n) {
+    size_t cc = g.c(n, p, k, ie, j);
+    st.du[cc] = dnext.a[n];
+    st.u[cc] += dnext.a[n];
+  }
+  double *cf = st.coef;
+  for (int i = ie - 1; i >= is; --i) {
+    M5 A;
+    V5 d;
+#pragma unroll
+    for (int r = 0; r < 5; ++r)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) A.a[r][q] = cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) d.a[n] = cf[(size_t)(25 + n) * g.vstride + cs30 + (size_t)i * g.jp];
+    V5 ad = m5_mulv(A, dnext);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
+    for (int n = 0; n < 5; ++n) {
+      size_t cc = g.c(n, p, k, i, j);
+      st.du[cc] = d.a[n];
+      st.u[cc] += d.a[n];
+    }
+    dnext = d;
+  }
+}
+
+// u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
+__global__ void k_add_du(Grid g, State st) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * N * ni * N;
+  if (t >= total) return;
+  int j = g.js + (int)(t % N);
+  size_t r = t / N;
+  int i = g.is + (int)(r % ni);
+  r /= ni;
+  int k = g.ks + (int)(r % N), p = (int)(r / N);
+  for (int n = 0; n < g.nv; ++n) {
+    size_t c = g.c(n, p, k, i, j);
+    st.u[c] += st.du[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Hydro::NewBlockTimeStep (patches/21.new_blockdt.patch:16-37): per-cell
+// width / signal speed, block minimum, device-wide minimum on the IEEE bits
+// (positive doubles order like unsigned integers).
+__global__ void k_dt_reset(State st) { *st.dtbits = 0x7FF0000000000000ull; }
+
+__global__ void k_dt(Grid g, State st) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * N * ni * N;
+  double m = HUGE_VAL;
+  if (t < total) {
+    int j = g.js + (int)(t % N);
+    size_t r = t / N;
+    int i = g.is + (int)(r % ni);
+    r /= ni;
+    int k = g.ks + (int)(r % N), p = (int)(r / N);
+    double wi[5];
+    for (int n = 0; n < g.nv; ++n) wi[n] = st.w[g.c(n, p, k, i, j)];
+    double cs = g.eos == EOS_SW ? sqrt(wi[IDN])
+                                : sqrt((1. + (g.gamma - 1) * 1. / 1.) * wi[IPR] / wi[IDN]);
+    double dt1 = g.dx1f[i];
+    double dt2 = g.x1v[i] * g.T(T_DX2A, p, k, j), dt3 = g.x1v[i] * g.T(T_DX3A, p, k, j);
+    double sp1 = fabs(wi[IVX]) + cs, sp2 = fabs(wi[IVY]) + cs, sp3 = fabs(wi[IVZ]) + cs;
+    if ((g.implicit & 1) && N > 1) dt1 /= g.cfl * fabs(wi[IVX]);
+    else dt1 /= sp1;
+    dt2 /= sp2;
+    dt3 /= sp3;
+    if (g.cfl_x1 && dt1 < m) m = dt1;
+    if (dt2 < m) m = dt2;
+    if (dt3 < m) m = dt3;
+  }
+  // wavefront then block minimum
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(m, off);
+    m = o < m ? o : m;
+  }
+  __shared__ double wmin[BLK / 64];
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) wmin[wv] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = wmin[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = wmin[q] < b ? wmin[q] : b;
+    atomicMin(st.dtbits, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+// dt = min(cfl * min_cells, 2 dt_prev) (Mesh::NewTimeStep growth limit)
+__global__ void k_dt_finish(State st, double cfl, double dt_fixed) {
+  if (dt_fixed > 0.0) {
+    st.dtv[0] = dt_fixed;
+    return;
+  }
+  double d = __longlong_as_double((long long)*st.dtbits) * cfl;
+  double dt = 2.0 * st.dtv[1];
+  if (d < dt) dt = d;
+  st.dtv[0] = dt;
+  st.dtv[1] = dt;
+}
+
+// single-face / scalar shims
+__global__ void k_riemann_face(Grid g, int dir, int p, int k, int j, const double *wl0,
+                               const double *wr0, double *flx) {
+  double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
+  for (int n = 0; n < g.nv; ++n) {
+    wl[n] = wl0[n];
+    wr[n] = wr0[n];
+  }
+  double cth, cosf = 0., sinf = 1., cosc = g.T(T_COSC, p, k, j);
+  if (dir == 0) cth = g.T(T_CTH1, p, k, j);
+  else if (dir == 1) { cth = g.T(T_CTH2, p, k, j); cosf = g.T(T_COS2, p, k, j); sinf = g.T(T_SIN2, p, k, j); }
+  else { cth = g.T(T_CTH3, p, k, j); cosf = g.T(T_COS3, p, k, j); sinf = g.T(T_SIN3, p, k, j); }
+  riemann<double>(g.eos, g.rs, dir == 0 ? IVX : (dir == 1 ? IVY : IVZ), g.gamma, wl, wr, fl,
+                  cth, cosf, sinf, cosc);
+  for (int n = 0; n < g.nv; ++n) flx[n] = fl[n];
+}
+
+__global__ void k_weno_batch(const double *in5, double *out, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double *a = in5 + 5 * (size_t)t;
+  out[t] = weno5(a[0], a[1], a[2], a[3], a[4]);
+}
+
+__global__ void k_math_batch(int op, const double *x, double y, double *out, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[t] = op == 0 ? dexp(x[t]) : (op == 1 ? dlog(x[t]) : dpow(x[t], y));
+}
+
+}  // namespace
+
+// ===========================================================================
+// handle
+struct exo_handle {
+  Setup setup;
+  Grid g;
+  State st;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void *> allocs;
+  GhostDesc *d_ghost = nullptr;
+  EdgeDesc *d_edge = nullptr;
+  double *d_x1 = nullptr, *d_tab = nullptr;
+  double dfloor, sfloor;
+  bool have_state = false;
+  // live per-kernel timing with HIP events on the handle's stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kind;  // kernel id of each (start, end) pair
+
+  void *dalloc(size_t bytes) {
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes));
+    HIPCHK(hipMemsetAsync(p, 0, bytes, stream));
+    allocs.push_back(p);
+    return p;
+  }
+  size_t cells_bytes(int nvar) const { return (size_t)nvar * g.vstride * sizeof(double); }
+  size_t face_bytes(int nvar) const { return (size_t)nvar * g.fvstride * sizeof(double); }
+};
+
+namespace {
+
+unsigned grid1(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
+
+// kernel ids of the profile (exo_profile_end)
+enum { KID_PRIM = 0, KID_GHOST, KID_FLUX_X1, KID_EDGE, KID_FLUX_X2, KID_FLUX_X3, KID_UPDATE,
+       KID_IMPLICIT, KID_DT, KID_COUNT };
+
+hipEvent_t next_event(exo_handle *h) {
+  if (h->ev_used == h->ev_pool.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    h->ev_pool.push_back(e);
+  }
+  return h->ev_pool[h->ev_used++];
+}
+
+// records an event pair around one launch when profiling is on
+struct ProfScope {
+  exo_handle *h;
+  ProfScope(exo_handle *hh, int kid) : h(hh) {
+    if (h->prof) {
+      h->ev_kind.push_back(kid);
+      HIPCHK(hipEventRecord(next_event(h), h->stream));
+    }
+  }
+  ~ProfScope() noexcept(false) {
+    if (h->prof) HIPCHK(hipEventRecord(next_event(h), h->stream));
+  }
+};
+
+void init_handle(exo_handle *h) {
+  Setup &s = h->setup;
+  Grid &g = h->g;
+  g.N = s.N;
+  g.nx1 = s.nx1;
+  g.nc1 = s.nc1;
+  g.nc2 = s.nc2;
+  g.nc3 = s.nc3;
+  g.jp = (s.nc2 + 1 + 7) / 8 * 8;
+  g.nv = s.nv;
+  g.nt = s.nt;
+  g.is = s.is; g.ie = s.ie; g.js = s.js; g.je = s.je; g.ks = s.ks; g.ke = s.ke;
+  g.eos = s.cfg.eos;
+  g.rs = s.cfg.rsolver;
+  g.coriolis = s.cfg.coriolis;
+  g.implicit = s.cfg.implicit_flag;
+  g.cfl_x1 = s.cfg.cfl_x1;
+  g.gamma = s.cfg.gamma;
+  g.grav = s.cfg.eos == EXO_EOS_IDEAL ? s.cfg.grav : 0.;
+  g.cfl = s.cfg.cfl;
+  g.plane = (size_t)g.nc1 * g.jp;
+  g.pstride = (size_t)g.nc3 * g.plane;
+  g.vstride = 6 * g.pstride;
+  g.fplane = (size_t)(g.nc1 + 1) * g.jp;
+  g.fpstride = (size_t)(g.nc3 + 1) * g.fplane;
+  g.fvstride = 6 * g.fpstride;
+  g.t2stride = (size_t)6 * g.nc3 * g.nc2;
+  h->dfloor = sqrt(1024 * (double)FLT_MIN);
+  h->sfloor = sqrt(1024 * (double)FLT_MIN);
+
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  // geometry
+  std::vector<double> x1(3 * (s.nc1 + 1), 0.);
+  for (int i = 0; i <= s.nc1; ++i) x1[i] = s.x1f[i];
+  for (int i = 0; i < s.nc1; ++i) x1[s.nc1 + 1 + i] = s.x1v[i];
+  for (int i = 0; i < s.nc1; ++i) x1[2 * (s.nc1 + 1) + i] = s.dx1f[i];
+  h->d_x1 = (double *)h->dalloc(x1.size() * sizeof(double));
+  HIPCHK(hipMemcpyAsync(h->d_x1, x1.data(), x1.size() * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  g.x1f = h->d_x1;
+  g.x1v = h->d_x1 + (s.nc1 + 1);
+  g.dx1f = h->d_x1 + 2 * (s.nc1 + 1);
+  h->d_tab = (double *)h->dalloc(s.tab2.size() * sizeof(double));
+  HIPCHK(hipMemcpyAsync(h->d_tab, s.tab2.data(), s.tab2.size() * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
+  g.tab = h->d_tab;
+  h->d_ghost = (GhostDesc *)h->dalloc(s.ghosts.size() * sizeof(GhostDesc));
+  HIPCHK(hipMemcpyAsync(h->d_ghost, s.ghosts.data(), s.ghosts.size() * sizeof(GhostDesc),
+                        hipMemcpyHostToDevice, h->stream));
+  h->d_edge = (EdgeDesc *)h->dalloc(s.edges.size() * sizeof(EdgeDesc));
+  HIPCHK(hipMemcpyAsync(h->d_edge, s.edges.data(), s.edges.size() * sizeof(EdgeDesc),
+                        hipMemcpyHostToDevice, h->stream));
+  // state
+  State &st = h->st;
+  std::memset(&st, 0, sizeof(st));
+  st.u = (double *)h->dalloc(h->cells_bytes(g.nv));
+  st.w = (double *)h->dalloc(h->cells_bytes(g.nv));
+  st.u1 = (double *)h->dalloc(h->cells_bytes(g.nv));
+  st.du = (double *)h->dalloc(h->cells_bytes(g.nv));
+  if (g.nt) {
+    st.s = (double *)h->dalloc(h->cells_bytes(g.nt));
+    st.r = (double *)h->dalloc(h->cells_bytes(g.nt));
+    st.s1 = (double *)h->dalloc(h->cells_bytes(g.nt));
+    for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
+  }
+  for (int d = 0; d < 3; ++d) st.flx[d] = (double *)h->dalloc(h->face_bytes(g.nv));
+  st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
+  if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
+  st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
+  st.dtv = (double *)h->dalloc(2 * sizeof(double));
+  double dt0[2] = {0., DBL_MAX * 0.4};
+  HIPCHK(hipMemcpyAsync(st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+void launch_fill_ghosts(exo_handle *h) {
+  Grid &g = h->g;
+  State &st = h->st;
+  int nd = (int)h->setup.ghosts.size();
+  {
+    ProfScope ps_(h, KID_GHOST);
+    hipLaunchKernelGGL(k_ghost, dim3(grid1((size_t)nd * g.nx1)), dim3(BLK), 0, h->stream, g,
+                       h->d_ghost, nd, st.u, g.nv, 1);
+  }
+  if (g.nt)
+    {
+      ProfScope ps_(h, KID_GHOST);
+      hipLaunchKernelGGL(k_ghost, dim3(grid1((size_t)nd * g.nx1)), dim3(BLK), 0, h->stream, g,
+                         h->d_ghost, nd, st.s, g.nt, 0);
+    }
+  {
+    ProfScope ps_(h, KID_PRIM);
+    hipLaunchKernelGGL(k_prim, dim3(grid1(g.nc2, 64), g.nc3, 6), dim3(64), 0, h->stream, g, st,
+                       h->dfloor, h->sfloor);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void launch_fluxes(exo_handle *h) {
+  Grid &g = h->g;
+  State &st = h->st;
+  if (g.eos == EXO_EOS_IDEAL) {
+    size_t lds = (size_t)(g.nc1 + 1) * 64 * sizeof(double);
+    {
+      ProfScope ps_(h, KID_FLUX_X1);
+      hipLaunchKernelGGL(k_flux_x1, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), lds,
+                         h->stream, g, st);
+    }
+  }
+  int ne = (int)h->setup.edges.size();
+  {
+    ProfScope ps_(h, KID_EDGE);
+    hipLaunchKernelGGL(k_edge_states, dim3(grid1((size_t)ne * g.nx1)), dim3(BLK), 0, h->stream,
+                       g, st, h->d_edge, ne);
+  }
+  size_t nf = (size_t)6 * g.N * g.nx1 * (g.N + 1);
+  {
+    ProfScope ps_(h, KID_FLUX_X2);
+    hipLaunchKernelGGL(k_flux_h, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st, 1);
+  }
+  {
+    ProfScope ps_(h, KID_FLUX_X3);
+    hipLaunchKernelGGL(k_flux_h, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st, 2);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+const double SS_DELTA[3] = {1.0, 0.0, 0.0};
+const double SS_G1[3] = {0.0, 0.25, 2.0 / 3.0};
+const double SS_G2[3] = {1.0, 0.75, 1.0 / 3.0};
+const double SS_BETA[3] = {1.0, 0.25, 2.0 / 3.0};
+
+void launch_update(exo_handle *h, int stage) {
+  Grid &g = h->g;
+  int st0 = stage - 1;
+  StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0};
+  size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
+  {
+    ProfScope ps_(h, KID_UPDATE);
+    hipLaunchKernelGGL(k_update, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void launch_implicit(exo_handle *h, double beta) {
+  Grid &g = h->g;
+  if (g.eos != EXO_EOS_IDEAL) return;
+  if (g.implicit) {
+    {
+      ProfScope ps_(h, KID_IMPLICIT);
+      hipLaunchKernelGGL(k_implicit, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), 0,
+                         h->stream, g, h->st, beta);
+    }
+  } else {
+    size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
+    {
+      ProfScope ps_(h, KID_IMPLICIT);
+      hipLaunchKernelGGL(k_add_du, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st);
+    }
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void launch_dt(exo_handle *h, double dt_fixed) {
+  Grid &g = h->g;
+  if (dt_fixed <= 0.0) {
+    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(1), 0, h->stream, h->st);
+    size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
+    {
+      ProfScope ps_(h, KID_DT);
+      hipLaunchKernelGGL(k_dt, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st);
+    }
+  }
+  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1), 0, h->stream, h->st, g.cfl, dt_fixed);
+  HIPCHK(hipGetLastError());
+}
+
+void run_stage(exo_handle *h, int stage) {
+  launch_fluxes(h);
+  launch_update(h, stage);
+  launch_implicit(h, SS_BETA[stage - 1]);
+  launch_fill_ghosts(h);
+}
+
+// copy between the host MeshBlock layout (p, n, k, j, i) and the device
+// layout (n, p, k, i, j)
+void host_to_dev(exo_handle *h, const double *src, double *dst, int nvar) {
+  Grid &g = h->g;
+  std::vector<double> buf((size_t)nvar * g.vstride, 0.);
+  for (int p = 0; p < 6; ++p)
+    for (int n = 0; n < nvar; ++n)
+      for (int k = 0; k < g.nc3; ++k)
+        for (int j = 0; j < g.nc2; ++j)
+          for (int i = 0; i < g.nc1; ++i)
+            buf[g.c(n, p, k, i, j)] =
+                src[((((size_t)p * nvar + n) * g.nc3 + k) * g.nc2 + j) * g.nc1 + i];
+  HIPCHK(hipMemcpyAsync(dst, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+}
+
+void dev_to_host(exo_handle *h, const double *src, double *dst, int nvar) {
+  Grid &g = h->g;
+  std::vector<double> buf((size_t)nvar * g.vstride);
+  HIPCHK(hipMemcpyAsync(buf.data(), src, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int p = 0; p < 6; ++p)
+    for (int n = 0; n < nvar; ++n)
+      for (int k = 0; k < g.nc3; ++k)
+        for (int j = 0; j < g.nc2; ++j)
+          for (int i = 0; i < g.nc1; ++i)
+            dst[((((size_t)p * nvar + n) * g.nc3 + k) * g.nc2 + j) * g.nc1 + i] =
+                buf[g.c(n, p, k, i, j)];
+}
+
+template <typename F>
+int guarded(exo_handle *h, F &&fn) {
+  if (!h) return EXO_ERR_ARG;
+  try {
+    fn();
+    return EXO_OK;
+  } catch (const std::string &m) {
+    h->err = m;
+    return m.rfind("HIP", 0) == 0 ? EXO_ERR_HIP : EXO_ERR_ARG;
+  } catch (const std::exception &e) {
+    h->err = e.what();
+    return EXO_ERR_NOMEM;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int exo_create(const exo_config *cfg, exo_handle **out) {
+  if (!cfg || !out) return EXO_ERR_ARG;
+  *out = nullptr;
+  exo_handle *h = new exo_handle();
+  std::string m = build_setup(*cfg, &h->setup);
+  if (!m.empty()) {
+    fprintf(stderr, "exo_create: %s\n", m.c_str());
+    delete h;
+    return EXO_ERR_ARG;
+  }
+  if (cfg->precision != EXO_PREC_FP64 && cfg->precision != EXO_PREC_MIXED) {
+    delete h;
+    return EXO_ERR_ARG;
+  }
+  try {
+    init_handle(h);
+  } catch (const std::string &e) {
+    fprintf(stderr, "exo_create: %s\n", e.c_str());
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return EXO_ERR_HIP;
+  }
+  *out = h;
+  return EXO_OK;
+}
+
+int exo_destroy(exo_handle *h) {
+  if (!h) return EXO_ERR_ARG;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (void *p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return EXO_OK;
+}
+
+const char *exo_last_error(const exo_handle *h) { return h ? h->err.c_str() : "null handle"; }
+int exo_nvar(const exo_handle *h) { return h ? h->g.nv : 0; }
+size_t exo_state_doubles(const exo_handle *h) {
+  return h ? (size_t)6 * h->g.nv * h->g.nc3 * h->g.nc2 * h->g.nc1 : 0;
+}
+
+int exo_upload_state(exo_handle *h, const double *u, const double *s) {
+  return guarded(h, [&] {
+    if (!u) throw std::string("null state");
+    host_to_dev(h, u, h->st.u, h->g.nv);
+    if (h->g.nt) {
+      if (!s) throw std::string("scalars required (ntracer > 0)");
+      host_to_dev(h, s, h->st.s, h->g.nt);
+    }
+    double dt0[2] = {0., DBL_MAX * 0.4};
+    HIPCHK(hipMemcpyAsync(h->st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
+    launch_fill_ghosts(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_state = true;
+  });
+}
+
+int exo_download_state(exo_handle *h, double *u, double *s) {
+  return guarded(h, [&] {
+    if (!u) throw std::string("null output");
+    dev_to_host(h, h->st.u, u, h->g.nv);
+    if (h->g.nt && s) dev_to_host(h, h->st.s, s, h->g.nt);
+  });
+}
+
+int exo_download_prim(exo_handle *h, double *w) {
+  return guarded(h, [&] {
+    if (!w) throw std::string("null output");
+    dev_to_host(h, h->st.w, w, h->g.nv);
+  });
+}
+
+int exo_calc_fluxes(exo_handle *h) {
+  return guarded(h, [&] {
+    if (!h->have_state) throw std::string("no state uploaded");
+    launch_fluxes(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_download_flux(exo_handle *h, int dir, double *out) {
+  return guarded(h, [&] {
+    if (dir < 0 || dir > 2 || !out) throw std::string("bad flux direction");
+    Grid &g = h->g;
+    std::vector<double> buf((size_t)g.nv * g.fvstride);
+    HIPCHK(hipMemcpyAsync(buf.data(), h->st.flx[dir], buf.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int n1 = g.nc1 + (dir == 0), n2 = g.nc2 + (dir == 1), n3 = g.nc3 + (dir == 2);
+    for (int p = 0; p < 6; ++p)
+      for (int n = 0; n < g.nv; ++n)
+        for (int k = 0; k < n3; ++k)
+          for (int j = 0; j < n2; ++j)
+            for (int i = 0; i < n1; ++i)
+              out[((((size_t)p * g.nv + n) * n3 + k) * n2 + j) * n1 + i] = buf[g.f(n, p, k, i, j)];
+  });
+}
+
+int exo_add_flux_divergence(exo_handle *h, double wght) {
+  return guarded(h, [&] {
+    // stage-1 weights with beta * dt = wght: u1 <- u, u <- u1, then the update
+    double dtv = wght;
+    HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    launch_update(h, 1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_fill_ghosts(exo_handle *h) {
+  return guarded(h, [&] {
+    launch_fill_ghosts(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_implicit_correct(exo_handle *h, double dt_stage) {
+  return guarded(h, [&] {
+    double dtv = dt_stage;
+    HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    launch_implicit(h, 1.0);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_new_dt(exo_handle *h, double *dt) {
+  return guarded(h, [&] {
+    launch_dt(h, 0.0);
+    double v[2];
+    HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (dt) *dt = v[0];
+  });
+}
+
+int exo_stage(exo_handle *h, int stage, double dt) {
+  return guarded(h, [&] {
+    if (stage < 1 || stage > 3) throw std::string("stage must be 1..3");
+    double dtv = dt;
+    HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    run_stage(h, stage);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
+  return guarded(h, [&] {
+    if (!h->have_state) throw std::string("no state uploaded");
+    for (int s = 0; s < nsteps; ++s) {
+      launch_dt(h, dt_fixed);
+      for (int stg = 1; stg <= 3; ++stg) run_stage(h, stg);
+    }
+    if (dt_last) {
+      double v[2];
+      HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      *dt_last = v[0];
+    }
+  });
+}
+
+int exo_profile_begin(exo_handle *h) {
+  return guarded(h, [&] {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->ev_used = 0;
+    h->ev_kind.clear();
+    h->prof = true;
+  });
+}
+
+int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds) {
+  return guarded(h, [&] {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->prof = false;
+    for (int q = 0; q < nkinds; ++q) {
+      if (ms) ms[q] = 0.;
+      if (counts) counts[q] = 0;
+    }
+    for (size_t e = 0; e < h->ev_kind.size(); ++e) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, h->ev_pool[2 * e], h->ev_pool[2 * e + 1]));
+      int kid = h->ev_kind[e];
+      if (kid < nkinds) {
+        if (ms) ms[kid] += t;
+        if (counts) counts[kid] += 1;
+      }
+    }
+    h->ev_used = 0;
+    h->ev_kind.clear();
+  });
+}
+
+int exo_sync(exo_handle *h) {
+  return guarded(h, [&] { HIPCHK(hipStreamSynchronize(h->stream)); });
+}
+
+int exo_init_state(exo_handle *h, int kind, uint64_t seed, const double *params, int nparams,
+                   double *u_out, double *s_out) {
+  return guarded(h, [&] {
+    if (!u_out) throw std::string("null output");
+    std::string m = init_state(h->setup, kind, seed, params, nparams, u_out, s_out);
+    if (!m.empty()) throw m;
+  });
+}
+
+int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *wl,
+                     const double *wr, double *flx) {
+  return guarded(h, [&] {
+    if (dir < 0 || dir > 2 || p < 0 || p > 5 || k < 0 || k >= h->g.nc3 || j < 0 ||
+        j >= h->g.nc2)
+      throw std::string("bad face");
+    double *d;
+    HIPCHK(hipMalloc(&d, 15 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(d, wl, h->g.nv * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(d + 5, wr, h->g.nv * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_riemann_face, dim3(1), dim3(1), 0, h->stream, h->g, dir, p, k, j, d,
+                       d + 5, d + 10);
+    HIPCHK(hipMemcpyAsync(flx, d + 10, h->g.nv * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(d));
+  });
+}
+
+int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n) {
+  return guarded(h, [&] {
+    if (n <= 0) return;
+    double *d;
+    HIPCHK(hipMalloc(&d, (size_t)6 * n * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(d, in5, (size_t)5 * n * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
+    hipLaunchKernelGGL(k_weno_batch, dim3(grid1(n)), dim3(BLK), 0, h->stream, d, d + 5 * (size_t)n,
+                       n);
+    HIPCHK(hipMemcpyAsync(out, d + 5 * (size_t)n, (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(d));
+  });
+}
+
+int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n) {
+  return guarded(h, [&] {
+    if (n <= 0) return;
+    if (op < 0 || op > 2) throw std::string("bad op");
+    double *d;
+    HIPCHK(hipMalloc(&d, (size_t)2 * n * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(d, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_math_batch, dim3(grid1(n)), dim3(BLK), 0, h->stream, op, d, y, d + n, n);
+    HIPCHK(hipMemcpyAsync(out, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(d));
+  });
+}
+
+int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, const double *params,
+                       int nparams, double *u_out, double *s_out) {
+  if (!cfg || !u_out) return EXO_ERR_ARG;
+  Setup s;
+  if (!build_setup(*cfg, &s).empty()) return EXO_ERR_ARG;
+  std::string m = init_state(s, kind, seed, params, nparams, u_out, s_out);
+  if (!m.empty()) {
+    fprintf(stderr, "exo_init_state_cfg: %s\n", m.c_str());
+    return EXO_ERR_ARG;
+  }
+  return EXO_OK;
+}
+
+// Host-only introspection (no device needed): the set-up tables a handle
+// would upload.  which: 0 ghost descriptors, 1 edge descriptors, 2 metric
+// tables [T_NTAB][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f of the six panels,
+// 5 x3f of the six panels.
+int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t *count) {
+  if (!cfg || !count) return EXO_ERR_ARG;
+  Setup s;
+  std::string m = build_setup(*cfg, &s);
+  if (!m.empty()) return EXO_ERR_ARG;
+  const void *src = nullptr;
+  size_t n = 0, esz = 0;
+  std::vector<double> tmp;
+  switch (which) {
+    case 0: src = s.ghosts.data(); n = s.ghosts.size(); esz = sizeof(GhostDesc); break;
+    case 1: src = s.edges.data(); n = s.edges.size(); esz = sizeof(EdgeDesc); break;
+    case 2: src = s.tab2.data(); n = s.tab2.size(); esz = sizeof(double); break;
+    case 3:
+      tmp = s.x1f;
+      tmp.insert(tmp.end(), s.x1v.begin(), s.x1v.end());
+      tmp.insert(tmp.end(), s.dx1f.begin(), s.dx1f.end());
+      src = tmp.data(); n = tmp.size(); esz = sizeof(double); break;
+    case 4:
+    case 5:
+      for (int p = 0; p < 6; ++p) {
+        auto &v = which == 4 ? s.x2f[p] : s.x3f[p];
+        tmp.insert(tmp.end(), v.begin(), v.end());
+      }
+      src = tmp.data(); n = tmp.size(); esz = sizeof(double); break;
+    default: return EXO_ERR_ARG;
+  }
+  *count = n;
+  if (buf) {
+    if (cap < n * esz) return EXO_ERR_ARG;
+    std::memcpy(buf, src, n * esz);
+  }
+  return EXO_OK;
+}
+
+}  // extern "C"

@@ -1835,3 +1835,79 @@ double or_face_area(const or_solver *s, int p, int dir, int k, int j, int i) {
 void or_latlon(const or_solver *s, int p, int k, int j, double *lat, double *lon) {
   cs_latlon(s, &s->pb[p], k, j, lat, lon);
 }
+
+/* ------------------------------------------------------------------------ */
+/* shims for the per-function pins (tests/test_oracle_pins.py)               */
+void or_flux_jacobian(double gm1, const double *w, double *out25) {
+  m5 J = flux_jacobian(gm1, w);
+  memcpy(out25, J.a, sizeof(J.a));
+}
+
+void or_eigensystem(const double *prim, double cs, double gm1, double *R25, double *Ri25,
+                    double *absA25) {
+  double r = prim[IDN], u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / r;
+  double h = hp + ke;
+  double R[5][5] = {{1., 1., 1., 0., 0.},
+                    {u - cs, u, u + cs, 0., 0.},
+                    {v, v, v, 1., 0.},
+                    {w, w, w, 0., 1.},
+                    {h - u * cs, ke, h + u * cs, v, w}};
+  double Ri[5][5] = {{(cs * ke + u * hp) / (2. * cs * hp), (-hp - cs * u) / (2. * cs * hp),
+                      -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+                     {(hp - ke) / hp, u / hp, v / hp, w / hp, -1. / hp},
+                     {(cs * ke - u * hp) / (2. * cs * hp), (hp - cs * u) / (2. * cs * hp),
+                      -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
+                     {-v, 0., 1., 0., 0.},
+                     {-w, 0., 0., 1., 0.}};
+  memcpy(R25, R, sizeof(R));
+  memcpy(Ri25, Ri, sizeof(Ri));
+  m5 A = abs_a(prim, cs, gm1);
+  memcpy(absA25, A.a, sizeof(A.a));
+}
+
+void or_roe_average(const double *wl, const double *wr, double gm1, double *prim) {
+  roe_average(prim, gm1, wl, wr);
+}
+
+/* block-tridiagonal solve of ForwardSweep / BackwardSubstitution on n rows:
+ * a, b, c: (n,5,5) row-major blocks (b[0], c[n-1] unused), rhs: (n,5) */
+void or_block_thomas(int n, const double *a_in, const double *b_in, const double *c_in,
+                     const double *rhs_in, double *x) {
+  m5 *a = malloc(sizeof(m5) * n), *b = malloc(sizeof(m5) * n), *c = malloc(sizeof(m5) * n);
+  v5 *d = malloc(sizeof(v5) * n);
+  memcpy(a, a_in, sizeof(m5) * n);
+  memcpy(b, b_in, sizeof(m5) * n);
+  memcpy(c, c_in, sizeof(m5) * n);
+  for (int i = 0; i < n; ++i) {
+    v5 rhs;
+    memcpy(rhs.a, rhs_in + 5 * i, sizeof(rhs.a));
+    if (i == 0) {
+      a[i] = m5_inv(&a[i]);
+      d[i] = m5_mulv(&a[i], &rhs);
+      a[i] = m5_mul(&a[i], &c[i]);
+    } else {
+      m5 ba = m5_mul(&b[i], &a[i - 1]), t;
+      for (int r = 0; r < 5; ++r)
+        for (int q = 0; q < 5; ++q) t.a[r][q] = a[i].a[r][q] - ba.a[r][q];
+      a[i] = m5_inv(&t);
+      v5 bd = m5_mulv(&b[i], &d[i - 1]), tv;
+      for (int q = 0; q < 5; ++q) tv.a[q] = rhs.a[q] - bd.a[q];
+      d[i] = m5_mulv(&a[i], &tv);
+      a[i] = m5_mul(&a[i], &c[i]);
+    }
+  }
+  for (int i = n - 2; i >= 0; --i) {
+    v5 ad = m5_mulv(&a[i], &d[i + 1]);
+    for (int q = 0; q < 5; ++q) d[i].a[q] -= ad.a[q];
+  }
+  for (int i = 0; i < n; ++i) memcpy(x + 5 * i, d[i].a, sizeof(d[i].a));
+  free(a); free(b); free(c); free(d);
+}
+
+void or_vec_maps(double X, double Y, int blockID, double V2, double V3, double *U, double *V,
+                 double *V2b, double *V3b) {
+  vec_rll_from_abp(X, Y, blockID, V2, V3, U, V);
+  vec_abp_from_rll(X, Y, blockID, *U, *V, V2b, V3b);
+}

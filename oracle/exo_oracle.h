@@ -104,6 +104,16 @@ void or_riemann_face(const or_config *cfg, int ivx, const double *wl,
                      const double *wr, double *flx, double cth_face,
                      double cos_face, double sin_face, double cos_cell);
 
+/* pins of the implicit-solve algebra and the vector maps */
+void or_flux_jacobian(double gm1, const double *w, double *out25);
+void or_eigensystem(const double *prim, double cs, double gm1, double *R25, double *Ri25,
+                    double *absA25);
+void or_roe_average(const double *wl, const double *wr, double gm1, double *prim);
+void or_block_thomas(int n, const double *a, const double *b, const double *c,
+                     const double *rhs, double *x);
+void or_vec_maps(double X, double Y, int blockID, double V2, double V3, double *U, double *V,
+                 double *V2b, double *V3b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,155 @@
+"""ctypes wrapper of the oracle library (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+EOS_SHALLOW_YZ, EOS_IDEAL = 0, 1
+
+
+class OrConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("nx1", C.c_int), ("eos", C.c_int), ("rsolver", C.c_int),
+        ("implicit_flag", C.c_int), ("ntracer", C.c_int), ("coriolis", C.c_int),
+        ("cfl_x1", C.c_int), ("x1min", C.c_double), ("x1max", C.c_double),
+        ("gamma", C.c_double), ("grav", C.c_double), ("omega", C.c_double),
+        ("cfl", C.c_double),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+_DP = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        V = C.c_void_p
+        sig = {
+            "or_weno5": (C.c_double, [C.c_double] * 5),
+            "or_weno3": (C.c_double, [C.c_double] * 3),
+            "or_exp": (C.c_double, [C.c_double]),
+            "or_log": (C.c_double, [C.c_double]),
+            "or_pow": (C.c_double, [C.c_double, C.c_double]),
+            "or_create": (V, [C.POINTER(OrConfig)]),
+            "or_destroy": (None, [V]),
+            "or_set_cons": (None, [V, _DP]),
+            "or_get_cons": (None, [V, _DP]),
+            "or_get_prim": (None, [V, _DP]),
+            "or_set_scalars": (None, [V, _DP]),
+            "or_get_scalars": (None, [V, _DP]),
+            "or_get_geom2d": (None, [V, C.c_int, _DP]),
+            "or_get_coord": (None, [V, C.c_int, C.c_int, _DP]),
+            "or_cell_volume": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "or_face_area": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+            "or_latlon": (None, [V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
+            "or_fill_ghosts": (None, [V]),
+            "or_cons_to_prim": (None, [V]),
+            "or_calc_fluxes": (None, [V]),
+            "or_get_flux": (None, [V, C.c_int, _DP]),
+            "or_new_dt": (C.c_double, [V]),
+            "or_stage": (None, [V, C.c_int, C.c_double]),
+            "or_step": (C.c_double, [V, C.c_int, C.c_double]),
+            "or_set_threads": (None, [C.c_int]),
+            "or_implicit_column": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
+                                          C.c_double]),
+            "or_flux_jacobian": (None, [C.c_double, _DP, _DP]),
+            "or_eigensystem": (None, [_DP, C.c_double, C.c_double, _DP, _DP, _DP]),
+            "or_roe_average": (None, [_DP, _DP, C.c_double, _DP]),
+            "or_block_thomas": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
+            "or_vec_maps": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "or_riemann_face": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP,
+                                       C.c_double, C.c_double, C.c_double, C.c_double]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def or_config(cfg) -> OrConfig:
+    """Build the oracle config from any object with the exo_config fields."""
+    d = cfg if isinstance(cfg, dict) else {f: getattr(cfg, f) for f, _ in OrConfig._fields_}
+    return OrConfig(**{f: d[f] for f, _ in OrConfig._fields_})
+
+
+class Oracle:
+    def __init__(self, cfg):
+        self.cfg = or_config(cfg)
+        self.L = lib()
+        self.h = self.L.or_create(C.byref(self.cfg))
+        self.nv = 4 if self.cfg.eos == EOS_SHALLOW_YZ else 5
+        self.nc1, self.nc2 = self.cfg.nx1 + 6, self.cfg.nx + 6
+        self.shape = (6, self.nv, self.nc2, self.nc2, self.nc1)
+        self.sshape = (6, self.cfg.ntracer, self.nc2, self.nc2, self.nc1)
+
+    def close(self):
+        if self.h:
+            self.L.or_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, u, s=None):
+        if s is not None and self.cfg.ntracer:
+            self.L.or_set_scalars(self.h, np.ascontiguousarray(s, dtype=np.float64))
+        self.L.or_set_cons(self.h, np.ascontiguousarray(u, dtype=np.float64))
+
+    def download(self):
+        u = np.zeros(self.shape)
+        self.L.or_get_cons(self.h, u)
+        s = None
+        if self.cfg.ntracer:
+            s = np.zeros(self.sshape)
+            self.L.or_get_scalars(self.h, s)
+        return u, s
+
+    def prim(self):
+        w = np.zeros(self.shape)
+        self.L.or_get_prim(self.h, w)
+        return w
+
+    def calc_fluxes(self):
+        self.L.or_calc_fluxes(self.h)
+
+    def flux(self, d):
+        n1 = self.nc1 + (d == 0)
+        n2 = self.nc2 + (d == 1)
+        n3 = self.nc2 + (d == 2)
+        out = np.zeros((6, self.nv, n3, n2, n1))
+        self.L.or_get_flux(self.h, d, out)
+        return out
+
+    def step(self, nsteps=1, dt=0.0):
+        return self.L.or_step(self.h, nsteps, dt)
+
+    def stage(self, stage, dt):
+        self.L.or_stage(self.h, stage, dt)
+
+    def new_dt(self):
+        return self.L.or_new_dt(self.h)

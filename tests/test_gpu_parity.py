@@ -1,0 +1,118 @@
+"""FP64 GPU path vs the oracle, through the C ABI.  The FP64 kernels keep the
+reference's operation order and are compiled without contraction, so the
+expected difference is zero; the pass bar is the 1e-12 relative bound."""
+import numpy as np
+import pytest
+
+from helpers import atm_config, interior, rel_err, sw_config
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _run_both(ex, orc, cfg, kind, nsteps, seed=20261202, params=()):
+    u0, s0 = ex.init_state(cfg, kind, seed, params)
+    g = ex.Solver(cfg)
+    g.upload(u0, s0)
+    o = orc.Oracle(cfg)
+    o.upload(u0, s0)
+    dg = g.step(nsteps)
+    do = o.step(nsteps)
+    ug, sg = g.download()
+    uo, so = o.download()
+    g.close()
+    o.close()
+    return dg, do, ug, uo, sg, so
+
+
+def test_weno5_device_matches_oracle(ex, orc):
+    cfg = sw_config(ex)
+    g = ex.Solver(cfg)
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.standard_normal((2000, 5)) * 10 ** rng.uniform(-8, 8, (2000, 1)),
+                        np.array([[1., 2., 3., 4., 5.]])])
+    out = np.zeros(len(x))
+    assert g.L.exo_weno5_batch(g.h, np.ascontiguousarray(x), out, len(x)) == 0
+    ref = np.array([orc.lib().or_weno5(*r) for r in x])
+    assert out[-1] == 2.5000000000000004
+    assert np.array_equal(out, ref)
+
+
+def test_math_device_matches_oracle(ex, orc):
+    cfg = sw_config(ex)
+    g = ex.Solver(cfg)
+    L = orc.lib()
+    x = np.exp(np.random.default_rng(2).uniform(-30, 30, 4000))
+    for op, f in [(0, lambda v: L.or_exp(v)), (1, lambda v: L.or_log(v)),
+                  (2, lambda v: L.or_pow(v, 1 / 1.4))]:
+        xx = np.log(x) if op == 0 else x
+        out = np.zeros(len(xx))
+        assert g.L.exo_math_batch(g.h, op, np.ascontiguousarray(xx), 1 / 1.4, out, len(xx)) == 0
+        ref = np.array([f(v) for v in xx])
+        assert np.array_equal(out, ref), op
+
+
+@pytest.mark.parametrize("rsolver", [0, 1])
+def test_shallow_water_fluxes(ex, orc, rsolver):
+    cfg = sw_config(ex, nx=8, rsolver=rsolver)
+    u0, _ = ex.init_state(cfg, 0, 3)
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    o = orc.Oracle(cfg)
+    o.upload(u0)
+    wg, wo = g.prim(), o.prim()
+    assert np.array_equal(interior(wg, 3), interior(wo, 3))
+    g.calc_fluxes()
+    o.calc_fluxes()
+    for d in (1, 2):
+        fg, fo = g.flux(d), o.flux(d)
+        if d == 1:
+            sl = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, 6))
+        else:
+            sl = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, 6))
+        assert np.array_equal(fg[sl], fo[sl]), (d, np.abs(fg[sl] - fo[sl]).max())
+
+
+@pytest.mark.parametrize("rsolver", [0, 1])
+def test_shallow_water_steps(ex, orc, rsolver):
+    cfg = sw_config(ex, nx=8, rsolver=rsolver)
+    dg, do, ug, uo, _, _ = _run_both(ex, orc, cfg, 0, 4)
+    assert dg == do
+    e = rel_err(interior(ug, 3), interior(uo, 3))
+    assert e.max() <= TOL, e
+
+
+@pytest.mark.parametrize("rsolver,implicit", [(3, 9), (2, 9), (3, 0)])
+def test_atmosphere_steps(ex, orc, rsolver, implicit):
+    cfg = atm_config(ex, nx=8, nx1=8, rsolver=rsolver, implicit=implicit)
+    nsteps = 3 if implicit else 1
+    dg, do, ug, uo, _, _ = _run_both(ex, orc, cfg, 1, nsteps)
+    assert dg == do
+    e = rel_err(interior(ug, 8), interior(uo, 8))
+    assert e.max() <= TOL, e
+
+
+def test_atmosphere_fluxes_x1(ex, orc):
+    cfg = atm_config(ex, nx=8, nx1=8)
+    u0, _ = ex.init_state(cfg, 1, 5)
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    o = orc.Oracle(cfg)
+    o.upload(u0)
+    g.calc_fluxes()
+    o.calc_fluxes()
+    sl = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, 3 + 9))
+    fg, fo = g.flux(0), o.flux(0)
+    assert np.array_equal(fg[sl], fo[sl]), np.abs(fg[sl] - fo[sl]).max()
+    for d in (1, 2):
+        sl = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, 11))
+        fg, fo = g.flux(d), o.flux(d)
+        assert np.array_equal(fg[sl], fo[sl]), (d, np.abs(fg[sl] - fo[sl]).max())
+
+
+def test_tracers_and_coriolis(ex, orc):
+    cfg = atm_config(ex, nx=8, nx1=8, rsolver=2, ntracer=2, coriolis=2)
+    dg, do, ug, uo, sg, so = _run_both(ex, orc, cfg, 2, 2)
+    assert dg == do
+    assert rel_err(interior(ug, 8), interior(uo, 8)).max() <= TOL
+    assert rel_err(interior(sg, 8), interior(so, 8)).max() <= TOL
