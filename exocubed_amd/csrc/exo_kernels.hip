@@ -18,6 +18,7 @@
 
 #include "../../include/exocubed.h"
 #include "exo_device.hpp"
+#include "exo_mixed.hpp"
 #include "exo_setup.hpp"
 
 using namespace exo;
@@ -76,6 +77,12 @@ struct State {
   double *coef;                  // implicit column coefficients, 30 per cell
   unsigned long long *dtbits;    // CFL minimum as ordered bits
   double *dtv;                   // [0] dt of the step, [1] previous dt
+  // mixed precision: FP32 perturbation fluxes, FP64 reference parts
+  float *fflx[3], *fsflx[3];     // face fluxes without the reference pressure
+  float *fedge;                  // ghost-side edge states (perturbations)
+  double *psf;                   // FP64 hydrostatic x1 face pressure [p][k][i][j]
+  double *lref;                  // per level: [0,nc1) split pressure P, [nc1,2nc1) the
+                                 // reconstructed variable's reference, [2nc1,3nc1) rho_ref
 };
 
 constexpr int BLK = 256;
@@ -230,7 +237,8 @@ __global__ void __launch_bounds__(64) k_flux_x1(Grid g, State st) {
   // window of 6 cells (f-3 .. f+2) for face f, perturbed IDN / IPR
   double q[6][5];
   auto load = [&](int slot, int i) {
-    for (int n = 0; n < nv; ++n) q[slot][n] = st.w[g.c(n, p, k, i, j)];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) q[slot][n] = n < nv ? st.w[g.c(n, p, k, i, j)] : 0.;
     if (decomp) {
       double a = PSF(i), b = PSF(i + 1), psv;
       if (fabs(a - b) < 1.E-6) psv = (a + b) / 2.;
@@ -265,7 +273,9 @@ __global__ void __launch_bounds__(64) k_flux_x1(Grid g, State st) {
       if (wl[IDN] < 0.) wl[IDN] = dsf;
     }
     riemann<double>(g.eos, g.rs, IVX, g.gamma, wl, wr, fl, cth, 0., 1., cosc);
-    for (int n = 0; n < nv; ++n) st.flx[0][g.f(n, p, k, fi, j)] = fl[n];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < nv) st.flx[0][g.f(n, p, k, fi, j)] = fl[n];
     if (fi <= ie) {
 #pragma unroll
       for (int s = 0; s < 5; ++s)
@@ -300,8 +310,10 @@ __global__ void k_edge_states(Grid g, State st, const EdgeDesc *__restrict__ ed,
   int e = t / ni, i = g.is + t % ni;
   EdgeDesc d = ed[e];
   int p = d.src_panel, k = d.src_k, j = d.src_j;
-  double q[5];
-  for (int n = 0; n < g.nv; ++n) {
+  double q[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
     double v;
     if (d.src_dir == 0)  // R state of cell js at face js
       v = weno5(st.w[g.c(n, p, k, i, j - 2)], st.w[g.c(n, p, k, i, j - 1)],
@@ -328,13 +340,16 @@ __global__ void k_edge_states(Grid g, State st, const EdgeDesc *__restrict__ ed,
   q[IVZ] = vy;
   size_t base = (((size_t)d.tgt_panel * 4 + d.tgt_dir) * g.N + d.tgt_a) * ni + (i - g.is);
   size_t vs = (size_t)6 * 4 * g.N * ni;
-  for (int n = 0; n < g.nv; ++n) st.edge[n * vs + base] = q[n];
+#pragma unroll
+  for (int n = 0; n < 5; ++n)
+    if (n < g.nv) st.edge[n * vs + base] = q[n];
 }
 
 // x2 faces (dir = 1) or x3 faces (dir = 2): WENO5 L/R of the face, the panel
 // edge's ghost side from the exchanged states, and Hydro::RiemannSolver with
 // the cubed-sphere frame maps.  One thread per (panel, k, i, j) face.
-__global__ void k_flux_h(Grid g, State st, int dir) {
+template <int dir>
+__global__ void k_flux_h(Grid g, State st) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   int N = g.N, ni = g.nx1;
   int na = N + 1;  // faces along the normal axis
@@ -361,8 +376,10 @@ __global__ void k_flux_h(Grid g, State st, int dir) {
   int k, j;
   if (dir == 1) { k = g.ks + kk; j = g.js + fa; }
   else { k = g.ks + fa; j = g.js + kk; }
-  double wl[5], wr[5], fl[5];
-  for (int n = 0; n < g.nv; ++n) {
+  double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
     double m3, m2, m1, c0, p1, p2;
     if (dir == 1) {
       m3 = st.w[g.c(n, p, k, i, j - 3)];
@@ -386,10 +403,14 @@ __global__ void k_flux_h(Grid g, State st, int dir) {
   int lo_side = dir == 1 ? 0 : 2;
   if (fa == 0) {  // ghost side is the L state
     size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
-    for (int n = 0; n < g.nv; ++n) wl[n] = st.edge[n * vs + b];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) wl[n] = st.edge[n * vs + b];
   } else if (fa == N) {  // ghost side is the R state
     size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
-    for (int n = 0; n < g.nv; ++n) wr[n] = st.edge[n * vs + b];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) wr[n] = st.edge[n * vs + b];
   }
   double cosf, sinf, cth;
   if (dir == 1) {
@@ -403,7 +424,9 @@ __global__ void k_flux_h(Grid g, State st, int dir) {
   }
   riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
   double *F = st.flx[dir];
-  for (int n = 0; n < g.nv; ++n) F[g.f(n, p, k, i, j)] = fl[n];
+#pragma unroll
+  for (int n = 0; n < 5; ++n)
+    if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
   for (int n = 0; n < g.nt; ++n) {
     double r[6];
     for (int s = 0; s < 6; ++s)
@@ -427,6 +450,7 @@ struct StageW {
   int first;
 };
 
+template <bool MIX>
 __global__ void k_update(Grid g, State st, StageW sw) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   int N = g.N, ni = g.nx1;
@@ -454,11 +478,30 @@ __global__ void k_update(Grid g, State st, StageW sw) {
     else u = sw.g1 * u + sw.g2 * u1;
     st.u1[c] = u1;
     double dflx;
-    if (g.eos == EOS_SW) dflx = 0.;
-    else
-      dflx = a1r * st.flx[0][g.f(n, p, k, i + 1, j)] - a1l * st.flx[0][g.f(n, p, k, i, j)];
-    dflx += a2r * st.flx[1][g.f(n, p, k, i, j + 1)] - a2l * st.flx[1][g.f(n, p, k, i, j)];
-    dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
+    if (!MIX) {
+      if (g.eos == EOS_SW) dflx = 0.;
+      else
+        dflx = a1r * st.flx[0][g.f(n, p, k, i + 1, j)] - a1l * st.flx[0][g.f(n, p, k, i, j)];
+      dflx += a2r * st.flx[1][g.f(n, p, k, i, j + 1)] - a2l * st.flx[1][g.f(n, p, k, i, j)];
+      dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
+    } else {
+      // FP32 perturbation fluxes plus the FP64 reference-pressure parts
+      const float *F0 = st.fflx[0], *F1 = st.fflx[1], *F2 = st.fflx[2];
+      if (g.eos == EOS_SW) dflx = 0.;
+      else
+        dflx = a1r * (double)F0[g.f(n, p, k, i + 1, j)] - a1l * (double)F0[g.f(n, p, k, i, j)];
+      dflx += a2r * (double)F1[g.f(n, p, k, i, j + 1)] - a2l * (double)F1[g.f(n, p, k, i, j)];
+      dflx += a3r * (double)F2[g.f(n, p, k + 1, i, j)] - a3l * (double)F2[g.f(n, p, k, i, j)];
+      double P = st.lref[i];
+      if (n == IVX && g.eos == EOS_IDEAL)
+        dflx += a1r * st.psf[g.f(0, p, k, i + 1, j)] - a1l * st.psf[g.f(0, p, k, i, j)];
+      if (n == IVY)
+        dflx += ((a2r * g.T(T_PRY2, p, k, j + 1) - a2l * g.T(T_PRY2, p, k, j)) +
+                 (a3r * g.T(T_PRY3, p, k + 1, j) - a3l * g.T(T_PRY3, p, k, j))) * P;
+      if (n == IVZ)
+        dflx += ((a2r * g.T(T_PRZ2, p, k, j + 1) - a2l * g.T(T_PRZ2, p, k, j)) +
+                 (a3r * g.T(T_PRZ3, p, k + 1, j) - a3l * g.T(T_PRZ3, p, k, j))) * P;
+    }
     if (tend) {
       st.u[c] = u;
       out[n] = 0.;
@@ -534,10 +577,20 @@ __global__ void k_update(Grid g, State st, StageW sw) {
     if (sw.first) s = s1;
     else s = sw.g1 * s + sw.g2 * s1;
     st.s1[c] = s1;
-    double dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
-    if (g.eos == EOS_SW) dflx = 0.;
-    dflx += a2r * st.sflx[1][g.f(n, p, k, i, j + 1)] - a2l * st.sflx[1][g.f(n, p, k, i, j)];
-    dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
+    double dflx;
+    if (!MIX) {
+      dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
+      if (g.eos == EOS_SW) dflx = 0.;
+      dflx += a2r * st.sflx[1][g.f(n, p, k, i, j + 1)] - a2l * st.sflx[1][g.f(n, p, k, i, j)];
+      dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
+    } else {
+      const float *S0 = st.fsflx[0], *S1 = st.fsflx[1], *S2 = st.fsflx[2];
+      dflx = g.eos == EOS_SW ? 0.
+                             : a1r * (double)S0[g.f(n, p, k, i + 1, j)] -
+                                   a1l * (double)S0[g.f(n, p, k, i, j)];
+      dflx += a2r * (double)S1[g.f(n, p, k, i, j + 1)] - a2l * (double)S1[g.f(n, p, k, i, j)];
+      dflx += a3r * (double)S2[g.f(n, p, k + 1, i, j)] - a3l * (double)S2[g.f(n, p, k, i, j)];
+    }
     s -= wght * dflx / vol;
     st.s[c] = s;
   }
@@ -673,6 +726,318 @@ __global__ void __launch_bounds__(64) k_implicit(Grid g, State st, double beta) 
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// FullCorrection with one column per lane group: 5 lanes per column (12
+// columns per wavefront), lane r owns row r of every 5x5 block.  Matrices a
+// lane needs whole (R^-1 of the Roe state, the block being inverted, c, the
+// previous a^-1 c) pass through LDS; the LU factorisation is repeated in every
+// lane of the group and lane r back-substitutes column r of the inverse.  The
+// arithmetic per entry is the same sequence as m5_mul / m5_inv, so the result
+// is identical to k_implicit.
+constexpr int IG = 5;                 // lanes per column
+constexpr int IGPW = 12;              // columns per 64-lane block
+constexpr int ILDS = 112;             // doubles of LDS per column group
+
+__device__ __forceinline__ double sel5(int r, double a0, double a1, double a2, double a3,
+                                       double a4) {
+  return r == 0 ? a0 : (r == 1 ? a1 : (r == 2 ? a2 : (r == 3 ? a3 : a4)));
+}
+
+// row r of R |Lambda| (flux_decomposition.hpp:63-96), given the full R^-1 in L
+__device__ __forceinline__ void absa_row(int r, const double *prim, double cs, double gm1,
+                                         double *Lri, double *out, bool write_ri) {
+  double u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR], rr = prim[IDN];
+  double ke = 0.5 * (u * u + v * v + w * w);
+  double hp = (gm1 + 1.) / gm1 * p / rr;
+  double h = hp + ke;
+  double lam0 = fabs(u - cs), lam1 = fabs(u), lam2 = fabs(u + cs), lam3 = fabs(u),
+         lam4 = fabs(u);
+  if (write_ri) {
+    // R^-1 row r (rows 3, 4 are constants but written all the same)
+    double ri[5];
+    if (r == 0) {
+      ri[0] = (cs * ke + u * hp) / (2. * cs * hp);
+      ri[1] = (-hp - cs * u) / (2. * cs * hp);
+      ri[2] = -v / (2. * hp);
+      ri[3] = -w / (2. * hp);
+      ri[4] = 1. / (2. * hp);
+    } else if (r == 1) {
+      ri[0] = (hp - ke) / hp;
+      ri[1] = u / hp;
+      ri[2] = v / hp;
+      ri[3] = w / hp;
+      ri[4] = -1. / hp;
+    } else if (r == 2) {
+      ri[0] = (cs * ke - u * hp) / (2. * cs * hp);
+      ri[1] = (hp - cs * u) / (2. * cs * hp);
+      ri[2] = -v / (2. * hp);
+      ri[3] = -w / (2. * hp);
+      ri[4] = 1. / (2. * hp);
+    } else if (r == 3) {
+      ri[0] = -v; ri[1] = 0.; ri[2] = 1.; ri[3] = 0.; ri[4] = 0.;
+    } else {
+      ri[0] = -w; ri[1] = 0.; ri[2] = 0.; ri[3] = 1.; ri[4] = 0.;
+    }
+    for (int q = 0; q < 5; ++q) Lri[r * 5 + q] = ri[q];
+    return;
+  }
+  double rl[5];
+  rl[0] = sel5(r, 1. * lam0, (u - cs) * lam0, v * lam0, w * lam0, (h - u * cs) * lam0);
+  rl[1] = sel5(r, 1. * lam1, u * lam1, v * lam1, w * lam1, ke * lam1);
+  rl[2] = sel5(r, 1. * lam2, (u + cs) * lam2, v * lam2, w * lam2, (h + u * cs) * lam2);
+  rl[3] = sel5(r, 0. * lam3, 0. * lam3, 1. * lam3, 0. * lam3, v * lam3);
+  rl[4] = sel5(r, 0. * lam4, 0. * lam4, 0. * lam4, 1. * lam4, w * lam4);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    double t = rl[0] * Lri[0 * 5 + c];
+#pragma unroll
+    for (int k = 1; k < 5; ++k) t = t + rl[k] * Lri[k * 5 + c];
+    out[c] = t;
+  }
+}
+
+// row r of FluxJacobian (flux_decomposition.hpp:102-120)
+__device__ __forceinline__ void jac_row(int r, double gm1, const double *w, double *out) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ], rho = w[IDN], pres = w[IPR];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
+  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
+  out[0] = sel5(r, 0, gm1 * s2 / 2 - v1 * v1, -v1 * v2, -v1 * v3, c1);
+  out[1] = sel5(r, 1., (2. - gm1) * v1, v2, v3, c2);
+  out[2] = sel5(r, 0., -gm1 * v2, v1, 0., -gm1 * v2 * v1);
+  out[3] = sel5(r, 0., -gm1 * v3, 0., v1, -gm1 * v3 * v1);
+  out[4] = sel5(r, 0., gm1, 0., 0., (gm1 + 1) * v1);
+}
+
+// LU with partial pivoting of the 5x5 in T (repeated per lane) and column
+// `col` of the inverse, m5_inv's arithmetic
+__device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) {
+  double lu[5][5];
+  int piv[5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) lu[r][c] = T[r * 5 + c];
+    piv[r] = r;
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    int p = c;
+    double best = fabs(lu[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < 5; ++r)
+      if (fabs(lu[r][c]) > best) {
+        best = fabs(lu[r][c]);
+        p = r;
+      }
+#pragma unroll
+    for (int r = c + 1; r < 5; ++r) {
+      if (p == r) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          double t = lu[c][q];
+          lu[c][q] = lu[r][q];
+          lu[r][q] = t;
+        }
+        int t = piv[c];
+        piv[c] = piv[r];
+        piv[r] = t;
+      }
+    }
+#pragma unroll
+    for (int r = c + 1; r < 5; ++r) {
+      double l = lu[r][c] / lu[c][c];
+      lu[r][c] = l;
+#pragma unroll
+      for (int q = c + 1; q < 5; ++q) lu[r][q] = lu[r][q] - l * lu[c][q];
+    }
+  }
+  double y[5];
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    double t = piv[r] == col ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < r; ++q) t = t - lu[r][q] * y[q];
+    y[r] = t;
+  }
+#pragma unroll
+  for (int r = 4; r >= 0; --r) {
+    double t = y[r];
+#pragma unroll
+    for (int q = r + 1; q < 5; ++q) t = t - lu[r][q] * xcol[q];
+    xcol[r] = t / lu[r][r];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_implicit_grp(Grid g, State st, double beta) {
+  __shared__ double lds[IGPW * ILDS];
+  int lane = threadIdx.x;
+  int grp = lane / IG, r = lane % IG;
+  bool lane_ok = grp < IGPW;
+  if (!lane_ok) { grp = 0; r = 0; }
+  int N = g.N;
+  int col = blockIdx.x * IGPW + grp;
+  bool act = lane_ok && col < N * N;
+  int p = blockIdx.y;
+  int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
+  double *L = lds + grp * ILDS;
+  double *Lri = L, *LT = L + 25, *LX = L + 50, *LC = L + 75, *Ltv = L + 100, *Ldp = L + 105;
+  // Aprev rows live in LT between levels is re-used; keep them separately:
+  __shared__ double lap[IGPW * 25];
+  double *LA = lap + grp * 25;
+  int is = g.is, ie = g.ie;
+  double dt = beta * st.dtv[0];
+  double gamma = g.gamma, grav = g.grav;
+  double cth = act ? g.T(T_COSC, p, k, j) : 0., sth = act ? g.T(T_SINC, p, k, j) : 1.;
+  auto wproj = [&](int i, double *w) {
+    for (int n = 0; n < 5; ++n) w[n] = act ? st.w[g.c(n, p, k, i, j)] : 1.;
+    w[IVY] += w[IVZ] * cth;
+    w[IVZ] *= sth;
+  };
+  double gm1c = (gamma - 1.) * 1. / 1.;
+  double gm1 = 0.5 * (gm1c + gm1c);
+  double wm[5], w0[5], wp[5], prim[5];
+  wproj(is - 1, wm);
+  wproj(is, w0);
+  roe_average(prim, gm1, wm, w0);
+  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  double Am[5], Ap[5];
+  if (lane_ok) absa_row(r, prim, cs, gm1, Lri, nullptr, true);
+  __syncthreads();
+  if (lane_ok) absa_row(r, prim, cs, gm1, Lri, Am, false);
+  __syncthreads();
+  size_t cs30 = act ? (size_t)g.c(0, p, k, 0, j) : 0;
+  for (int i = is; i <= ie; ++i) {
+    wproj(i + 1, wp);
+    roe_average(prim, gm1, w0, wp);
+    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    if (lane_ok) absa_row(r, prim, cs, gm1, Lri, nullptr, true);
+    __syncthreads();
+    if (lane_ok) absa_row(r, prim, cs, gm1, Lri, Ap, false);
+    double Jm[5], J0[5], Jp[5];
+    jac_row(r, gm1c, wm, Jm);
+    jac_row(r, gm1c, w0, J0);
+    jac_row(r, gm1c, wp, Jp);
+    double aleft = act ? g.area1(p, k, j, i) : 1., aright = act ? g.area1(p, k, j, i + 1) : 1.;
+    double v = act ? g.volume(p, k, j, i) : 1.;
+    double a[5], b[5], c[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double dtq = r == q ? 1. / dt : 0.;
+      double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
+      a[q] = (Am[q] * aleft + Ap[q] * aright + (aright - aleft) * J0[q]) / (2. * v) + dtq - phi;
+      b[q] = -(Am[q] + Jm[q]) * aleft / (2. * v);
+      c[q] = -(Ap[q] - Jp[q]) * aright / (2. * v);
+    }
+    if (i == is)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) a[q] += b[q] * (q == IVX ? -1. : 1.);
+    if (i == ie)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) a[q] += c[q] * (q == IVX ? -1. : 1.);
+    double rhs = act ? st.du[g.c(r, p, k, i, j)] / dt : 0.;
+    double T[5], tv;
+    if (i == is) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) T[q] = a[q];
+      tv = rhs;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double t = b[0] * LA[0 * 5 + q];
+#pragma unroll
+        for (int kk = 1; kk < 5; ++kk) t = t + b[kk] * LA[kk * 5 + q];
+        T[q] = a[q] - t;
+      }
+      double t = b[0] * Ldp[0];
+#pragma unroll
+      for (int kk = 1; kk < 5; ++kk) t = t + b[kk] * Ldp[kk];
+      tv = rhs - t;
+    }
+    if (lane_ok) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        LT[r * 5 + q] = T[q];
+        LC[r * 5 + q] = c[q];
+      }
+      Ltv[r] = tv;
+    }
+    __syncthreads();
+    double xcol[5];
+    inv_col(LT, r, xcol);
+    if (lane_ok)
+#pragma unroll
+      for (int q = 0; q < 5; ++q) LX[q * 5 + r] = xcol[q];
+    __syncthreads();
+    double xr[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) xr[q] = LX[r * 5 + q];
+    double d = xr[0] * Ltv[0];
+#pragma unroll
+    for (int kk = 1; kk < 5; ++kk) d = d + xr[kk] * Ltv[kk];
+    double An[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double t = xr[0] * LC[0 * 5 + q];
+#pragma unroll
+      for (int kk = 1; kk < 5; ++kk) t = t + xr[kk] * LC[kk * 5 + q];
+      An[q] = t;
+    }
+    __syncthreads();  // everyone has read LA / Ldp / LX / LC of this level
+    if (lane_ok) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) LA[r * 5 + q] = An[q];
+      Ldp[r] = d;
+    }
+    if (act) {
+      double *cf = st.coef;
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp] = An[q];
+      cf[(size_t)(25 + r) * g.vstride + cs30 + (size_t)i * g.jp] = d;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Am[q] = Ap[q];
+    for (int n = 0; n < 5; ++n) {
+      wm[n] = w0[n];
+      w0[n] = wp[n];
+    }
+    __syncthreads();
+  }
+  // backward substitution: delta(ie) is final; Ldp holds it
+  double dn = Ldp[r];
+  if (act) {
+    size_t cc = g.c(r, p, k, ie, j);
+    st.du[cc] = dn;
+    st.u[cc] += dn;
+  }
+  double *cf = st.coef;
+  for (int i = ie - 1; i >= is; --i) {
+    double A[5], d = 0.;
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) A[q] = cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp];
+      d = cf[(size_t)(25 + r) * g.vstride + cs30 + (size_t)i * g.jp];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) A[q] = 0.;
+    }
+    double t = A[0] * Ldp[0];
+#pragma unroll
+    for (int kk = 1; kk < 5; ++kk) t = t + A[kk] * Ldp[kk];
+    d = d - t;
+    __syncthreads();
+    if (lane_ok) Ldp[r] = d;
+    if (act) {
+      size_t cc = g.c(r, p, k, i, j);
+      st.du[cc] = d;
+      st.u[cc] += d;
+    }
+    __syncthreads();
+  }
+}
+
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
 __global__ void k_add_du(Grid g, State st) {
   size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -751,6 +1116,243 @@ __global__ void k_dt_finish(State st, double cfl, double dt_fixed) {
   st.dtv[1] = dt;
 }
 
+
+// ===========================================================================
+// Mixed precision (EXO_PREC_MIXED): FP32 WENO5 + Riemann on perturbations
+// about a reference that is constant along the sweep, the reference-pressure
+// part of the momentum flux removed (exo_mixed.hpp) and added back in FP64 by
+// k_update<true>.  The decomposition scan, metric terms, sources, the
+// implicit solve and the state stay FP64.
+
+// x1 fluxes: FP64 decomposition (psf in LDS, stored for the update), FP32
+// reconstruction of the perturbation and FP32 Riemann flux.
+__global__ void __launch_bounds__(64) k_flux_x1_mix(Grid g, State st) {
+#pragma clang fp contract(fast)
+  extern __shared__ double lds[];
+  int lane = threadIdx.x;
+  int col = blockIdx.x * 64 + lane;
+  int p = blockIdx.y;
+  int N = g.N;
+  if (col >= N * N) return;
+  int k = g.ks + col / N, j = g.js + col % N;
+  double *psf = lds;
+#define PSF(ii) psf[(ii) * 64 + lane]
+  int is = g.is, ie = g.ie;
+  double grav = -g.grav;
+  double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
+  PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * (pie / die)));
+  for (int i = ie; i >= is; --i)
+    PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
+  for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
+  for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
+  double ent0 = g.gamma;
+  double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
+  double ig = 1. / ent0, e1 = dexp(-ent1 / ent0);
+  float q[6][5];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    int i = is - 3 + s;
+    double a = PSF(i), b = PSF(i + 1);
+    double psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
+    q[s][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - dpow(psv, ig) * e1);
+    q[s][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
+    q[s][IVY] = (float)st.w[g.c(IVY, p, k, i, j)];
+    q[s][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
+    q[s][IPR] = (float)(st.w[g.c(IPR, p, k, i, j)] - psv);
+  }
+  float cth = (float)g.T(T_CTH1, p, k, j), cosc = (float)g.T(T_COSC, p, k, j);
+  float gamma = (float)g.gamma;
+  for (int fi = is; fi <= ie + 1; ++fi) {
+    float wl[5], wr[5], fl[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      wl[n] = weno5f(q[4][n], q[3][n], q[2][n], q[1][n], q[0][n]);
+      wr[n] = weno5f(q[1][n], q[2][n], q[3][n], q[4][n], q[5][n]);
+    }
+    double pf = PSF(fi);
+    float P = (float)pf;
+    // RestoreFromPerturbation clamps (change_to_perturbation.cpp:174-189)
+    if (wl[IPR] < -P) wl[IPR] = 0.f;
+    if (wr[IPR] < -P) wr[IPR] = 0.f;
+    float dsf = (float)(dpow(pf, ig) * e1);
+    wl[IDN] += dsf;
+    if (wl[IDN] < 0.f) wl[IDN] = dsf;
+    wr[IDN] += dsf;
+    if (wr[IDN] < 0.f) wr[IDN] = dsf;
+    riemann_pert(g.eos, g.rs, IVX, gamma, wl, wr, P, fl, cth, 0.f, 1.f, cosc);
+    for (int n = 0; n < 5; ++n) st.fflx[0][g.f(n, p, k, fi, j)] = fl[n];
+    st.psf[g.f(0, p, k, fi, j)] = pf;
+    if (fi <= ie) {
+#pragma unroll
+      for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int n = 0; n < 5; ++n) q[s][n] = q[s + 1][n];
+      int i = fi + 3;
+      double a = PSF(i), b = PSF(i + 1);
+      double psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
+      q[5][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - dpow(psv, ig) * e1);
+      q[5][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
+      q[5][IVY] = (float)st.w[g.c(IVY, p, k, i, j)];
+      q[5][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
+      q[5][IPR] = (float)(st.w[g.c(IPR, p, k, i, j)] - psv);
+    }
+  }
+  for (int n = 0; n < g.nt; ++n)
+    for (int fi = is; fi <= ie + 1; ++fi) {
+      float r[6];
+      for (int s = 0; s < 6; ++s) r[s] = (float)st.r[g.c(n, p, k, fi - 3 + s, j)];
+      float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
+      float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
+      float m = st.fflx[0][g.f(IDN, p, k, fi, j)];
+      st.fsflx[0][g.f(n, p, k, fi, j)] = m >= 0.f ? m * rl : m * rr;
+    }
+#undef PSF
+}
+
+// cell value as an FP32 perturbation about the level reference
+__device__ __forceinline__ float pert(const Grid &g, const State &st, int n, int p, int k,
+                                      int i, int j) {
+  double v = st.w[g.c(n, p, k, i, j)];
+  if (g.eos == EOS_SW) {
+    if (n == IDN) v -= st.lref[g.nc1 + i];
+  } else {
+    if (n == IPR) v -= st.lref[g.nc1 + i];
+    else if (n == IDN) v -= st.lref[2 * g.nc1 + i];
+  }
+  return (float)v;
+}
+
+__global__ void k_edge_states_mix(Grid g, State st, const EdgeDesc *__restrict__ ed,
+                                  int nedge) {
+#pragma clang fp contract(fast)
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int ni = g.nx1;
+  if (t >= nedge * ni) return;
+  int e = t / ni, i = g.is + t % ni;
+  EdgeDesc d = ed[e];
+  int p = d.src_panel, k = d.src_k, j = d.src_j;
+  float q[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
+    float v;
+    if (d.src_dir == 0)
+      v = weno5f(pert(g, st, n, p, k, i, j - 2), pert(g, st, n, p, k, i, j - 1),
+                 pert(g, st, n, p, k, i, j), pert(g, st, n, p, k, i, j + 1),
+                 pert(g, st, n, p, k, i, j + 2));
+    else if (d.src_dir == 1)
+      v = weno5f(pert(g, st, n, p, k, i, j + 1), pert(g, st, n, p, k, i, j),
+                 pert(g, st, n, p, k, i, j - 1), pert(g, st, n, p, k, i, j - 2),
+                 pert(g, st, n, p, k, i, j - 3));
+    else if (d.src_dir == 2)
+      v = weno5f(pert(g, st, n, p, k - 2, i, j), pert(g, st, n, p, k - 1, i, j),
+                 pert(g, st, n, p, k, i, j), pert(g, st, n, p, k + 1, i, j),
+                 pert(g, st, n, p, k + 2, i, j));
+    else
+      v = weno5f(pert(g, st, n, p, k + 1, i, j), pert(g, st, n, p, k, i, j),
+                 pert(g, st, n, p, k - 1, i, j), pert(g, st, n, p, k - 2, i, j),
+                 pert(g, st, n, p, k - 3, i, j));
+    q[n] = v;
+  }
+  double U, V, vx, vy;
+  vec_rll_from_abp(d.x2s, d.x3s, d.src_block, (double)q[IVY], (double)q[IVZ], &U, &V);
+  vec_abp_from_rll(d.x2d, d.x3d, d.tgt_block, U, V, &vx, &vy);
+  q[IVY] = (float)vx;
+  q[IVZ] = (float)vy;
+  size_t base = (((size_t)d.tgt_panel * 4 + d.tgt_dir) * g.N + d.tgt_a) * ni + (i - g.is);
+  size_t vs = (size_t)6 * 4 * g.N * ni;
+#pragma unroll
+  for (int n = 0; n < 5; ++n)
+    if (n < g.nv) st.fedge[n * vs + base] = q[n];
+}
+
+template <int dir>
+__global__ void k_flux_h_mix(Grid g, State st) {
+#pragma clang fp contract(fast)
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  int na = N + 1;
+  size_t total = (size_t)6 * N * ni * na;
+  if (t >= total) return;
+  int kk, fa, i, p;
+  if (dir == 1) {
+    fa = (int)(t % na);
+    size_t r = t / na;
+    i = g.is + (int)(r % ni);
+    r /= ni;
+    kk = (int)(r % N);
+    p = (int)(r / N);
+  } else {
+    kk = (int)(t % N);
+    size_t r = t / N;
+    i = g.is + (int)(r % ni);
+    r /= ni;
+    fa = (int)(r % na);
+    p = (int)(r / na);
+  }
+  int k = dir == 1 ? g.ks + kk : g.ks + fa;
+  int j = dir == 1 ? g.js + fa : g.js + kk;
+  float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
+    float v[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+      v[s] = dir == 1 ? pert(g, st, n, p, k, i, j - 3 + s) : pert(g, st, n, p, k - 3 + s, i, j);
+    wl[n] = weno5f(v[4], v[3], v[2], v[1], v[0]);
+    wr[n] = weno5f(v[1], v[2], v[3], v[4], v[5]);
+  }
+  size_t vs = (size_t)6 * 4 * N * ni;
+  int lo_side = dir == 1 ? 0 : 2;
+  if (fa == 0) {
+    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) wl[n] = st.fedge[n * vs + b];
+  } else if (fa == N) {
+    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) wr[n] = st.fedge[n * vs + b];
+  }
+  // full density for the ideal gas; shallow water keeps h' (the solver adds H)
+  float P;
+  if (g.eos == EOS_SW) {
+    P = (float)st.lref[g.nc1 + i];
+  } else {
+    float rref = (float)st.lref[2 * g.nc1 + i];
+    wl[IDN] += rref;
+    wr[IDN] += rref;
+    P = (float)st.lref[g.nc1 + i];
+  }
+  float cosf, sinf, cth;
+  if (dir == 1) {
+    cosf = (float)g.T(T_COS2, p, k, j);
+    sinf = (float)g.T(T_SIN2, p, k, j);
+    cth = (float)g.T(T_CTH2, p, k, j);
+  } else {
+    cosf = (float)g.T(T_COS3, p, k, j);
+    sinf = (float)g.T(T_SIN3, p, k, j);
+    cth = (float)g.T(T_CTH3, p, k, j);
+  }
+  riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf,
+               sinf, 0.f);
+  float *F = st.fflx[dir];
+#pragma unroll
+  for (int n = 0; n < 5; ++n)
+    if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
+  for (int n = 0; n < g.nt; ++n) {
+    float r[6];
+    for (int s = 0; s < 6; ++s)
+      r[s] = (float)(dir == 1 ? st.r[g.c(n, p, k, i, j - 3 + s)] : st.r[g.c(n, p, k - 3 + s, i, j)]);
+    float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
+    float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
+    float m = fl[IDN];
+    st.fsflx[dir][g.f(n, p, k, i, j)] = m >= 0.f ? m * rl : m * rr;
+  }
+}
+
 // single-face / scalar shims
 __global__ void k_riemann_face(Grid g, int dir, int p, int k, int j, const double *wl0,
                                const double *wr0, double *flx) {
@@ -797,6 +1399,8 @@ struct exo_handle {
   double *d_x1 = nullptr, *d_tab = nullptr;
   double dfloor, sfloor;
   bool have_state = false;
+  bool mixed = false;
+  bool implicit_per_lane = false;  // EXO_IMPLICIT_PER_LANE=1: one column per lane
   // live per-kernel timing with HIP events on the handle's stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -913,6 +1517,21 @@ void init_handle(exo_handle *h) {
   for (int d = 0; d < 3; ++d) st.flx[d] = (double *)h->dalloc(h->face_bytes(g.nv));
   st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
   if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
+  h->mixed = s.cfg.precision == EXO_PREC_MIXED;
+  {
+    const char *e = getenv("EXO_IMPLICIT_PER_LANE");
+    h->implicit_per_lane = e && e[0] == '1';
+  }
+  if (h->mixed) {
+    size_t fb = (size_t)g.fvstride * sizeof(float);
+    for (int d = 0; d < 3; ++d) {
+      st.fflx[d] = (float *)h->dalloc(fb * g.nv);
+      if (g.nt) st.fsflx[d] = (float *)h->dalloc(fb * g.nt);
+    }
+    st.fedge = (float *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(float));
+    if (g.eos == EXO_EOS_IDEAL) st.psf = (double *)h->dalloc(g.fvstride * sizeof(double));
+    st.lref = (double *)h->dalloc((size_t)3 * g.nc1 * sizeof(double));
+  }
   st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
   st.dtv = (double *)h->dalloc(2 * sizeof(double));
   double dt0[2] = {0., DBL_MAX * 0.4};
@@ -946,6 +1565,31 @@ void launch_fill_ghosts(exo_handle *h) {
 void launch_fluxes(exo_handle *h) {
   Grid &g = h->g;
   State &st = h->st;
+  if (h->mixed) {
+    if (g.eos == EXO_EOS_IDEAL) {
+      size_t lds = (size_t)(g.nc1 + 1) * 64 * sizeof(double);
+      ProfScope ps_(h, KID_FLUX_X1);
+      hipLaunchKernelGGL(k_flux_x1_mix, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), lds,
+                         h->stream, g, st);
+    }
+    int ne = (int)h->setup.edges.size();
+    {
+      ProfScope ps_(h, KID_EDGE);
+      hipLaunchKernelGGL(k_edge_states_mix, dim3(grid1((size_t)ne * g.nx1)), dim3(BLK), 0,
+                         h->stream, g, st, h->d_edge, ne);
+    }
+    size_t nf = (size_t)6 * g.N * g.nx1 * (g.N + 1);
+    {
+      ProfScope ps_(h, KID_FLUX_X2);
+      hipLaunchKernelGGL(k_flux_h_mix<1>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
+    }
+    {
+      ProfScope ps_(h, KID_FLUX_X3);
+      hipLaunchKernelGGL(k_flux_h_mix<2>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
+    }
+    HIPCHK(hipGetLastError());
+    return;
+  }
   if (g.eos == EXO_EOS_IDEAL) {
     size_t lds = (size_t)(g.nc1 + 1) * 64 * sizeof(double);
     {
@@ -963,11 +1607,11 @@ void launch_fluxes(exo_handle *h) {
   size_t nf = (size_t)6 * g.N * g.nx1 * (g.N + 1);
   {
     ProfScope ps_(h, KID_FLUX_X2);
-    hipLaunchKernelGGL(k_flux_h, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st, 1);
+    hipLaunchKernelGGL(k_flux_h<1>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
   }
   {
     ProfScope ps_(h, KID_FLUX_X3);
-    hipLaunchKernelGGL(k_flux_h, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st, 2);
+    hipLaunchKernelGGL(k_flux_h<2>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
   }
   HIPCHK(hipGetLastError());
 }
@@ -984,7 +1628,10 @@ void launch_update(exo_handle *h, int stage) {
   size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
   {
     ProfScope ps_(h, KID_UPDATE);
-    hipLaunchKernelGGL(k_update, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
+    if (h->mixed)
+      hipLaunchKernelGGL(k_update<true>, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
+    else
+      hipLaunchKernelGGL(k_update<false>, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
   }
   HIPCHK(hipGetLastError());
 }
@@ -995,8 +1642,12 @@ void launch_implicit(exo_handle *h, double beta) {
   if (g.implicit) {
     {
       ProfScope ps_(h, KID_IMPLICIT);
-      hipLaunchKernelGGL(k_implicit, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), 0,
-                         h->stream, g, h->st, beta);
+      if (h->implicit_per_lane)
+        hipLaunchKernelGGL(k_implicit, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), 0,
+                           h->stream, g, h->st, beta);
+      else
+        hipLaunchKernelGGL(k_implicit_grp, dim3(grid1((size_t)g.N * g.N, IGPW), 6), dim3(64), 0,
+                           h->stream, g, h->st, beta);
     }
   } else {
     size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
@@ -1059,6 +1710,45 @@ void dev_to_host(exo_handle *h, const double *src, double *dst, int nvar) {
           for (int i = 0; i < g.nc1; ++i)
             dst[((((size_t)p * nvar + n) * g.nc3 + k) * g.nc2 + j) * g.nc1 + i] =
                 buf[g.c(n, p, k, i, j)];
+}
+
+// Per-level reference of the mixed path: the horizontal mean of the uploaded
+// state at each level (pressure and density for the ideal gas, geopotential
+// for shallow water).  Any level-constant reference gives the same fluxes up to
+// FP32 rounding; a close one keeps the perturbations small.
+void set_reference(exo_handle *h, const double *u) {
+  Grid &g = h->g;
+  int nv = g.nv, nc1 = g.nc1, nc2 = g.nc2, nc3 = g.nc3;
+  std::vector<double> ref(3 * (size_t)nc1, 0.), pr(nc1, 0.), rh(nc1, 0.);
+  double gm1 = g.gamma - 1.0;
+  for (int i = g.is; i <= g.ie; ++i) {
+    double sp = 0., sr = 0.;
+    for (int p = 0; p < 6; ++p)
+      for (int k = g.ks; k <= g.ke; ++k)
+        for (int j = g.js; j <= g.je; ++j) {
+          auto U = [&](int n) { return u[((((size_t)p * nv + n) * nc3 + k) * nc2 + j) * nc1 + i]; };
+          double d = U(IDN);
+          sr += d;
+          if (g.eos == EOS_IDEAL)
+            sp += gm1 * (U(IEN) - 0.5 * (U(IVX) * U(IVX) + U(IVY) * U(IVY) + U(IVZ) * U(IVZ)) / d);
+        }
+    double cnt = 6.0 * g.N * g.N;
+    rh[i] = sr / cnt;
+    pr[i] = sp / cnt;
+  }
+  for (int i = 0; i < nc1; ++i) {
+    int ii = std::min(std::max(i, g.is), g.ie);
+    if (g.eos == EOS_SW) {
+      ref[i] = 0.5 * rh[ii] * rh[ii];  // split "pressure" h^2/2
+      ref[nc1 + i] = rh[ii];           // geopotential reference
+    } else {
+      ref[i] = pr[ii];
+      ref[nc1 + i] = pr[ii];
+      ref[2 * nc1 + i] = rh[ii];
+    }
+  }
+  HIPCHK(hipMemcpyAsync(h->st.lref, ref.data(), ref.size() * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
 }
 
 template <typename F>
@@ -1135,6 +1825,7 @@ int exo_upload_state(exo_handle *h, const double *u, const double *s) {
     }
     double dt0[2] = {0., DBL_MAX * 0.4};
     HIPCHK(hipMemcpyAsync(h->st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
+    if (h->mixed) set_reference(h, u);
     launch_fill_ghosts(h);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_state = true;

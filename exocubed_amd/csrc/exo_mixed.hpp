@@ -1,0 +1,203 @@
+// exo_mixed.hpp -- FP32 face fluxes in perturbation form for the mixed-precision
+// path.  The face states carry the pressure (shallow water: the geopotential)
+// as a perturbation p' about a reference P that is constant along the sweep
+// (a per-level reference for x2 / x3 faces, the FP64 hydrostatic face pressure
+// psf of the decomposition for x1 faces).  The solvers below return the flux
+// with the reference-pressure part of the normal momentum flux removed; the
+// FP64 update kernel adds P times the face's metric factors back, so the
+// large hydrostatic and metric pressure balances never pass through FP32.
+// The algebra follows the FP64 solvers in exo_device.hpp (and the reference
+// files they cite) with the pressure split out exactly:
+//   LMARS:  pbar = P + pbar',  pbar' = (p'_l + p'_r)/2 + rho c (u_l - u_r)/2
+//   HLLC:   t_l = P + t_l', cp = P + cp', sl + sr + sm = 1
+//   Roe/LMARS shallow water: (h^2)/2 about h_ref.
+#pragma once
+
+#include "exo_device.hpp"
+
+namespace exo {
+
+EXO_HD float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+// WENO5 of exo_device.hpp in FP32 with hardware reciprocals for the weights
+EXO_HD float weno5f(float phim2, float phim1, float phi, float phip1, float phip2) {
+  float p0 = (1.f / 3.f) * phi + (5.f / 6.f) * phim1 - (1.f / 6.f) * phim2;
+  float p1 = (-1.f / 6.f) * phip1 + (5.f / 6.f) * phi + (1.f / 3.f) * phim1;
+  float p2 = (1.f / 3.f) * phip2 - (7.f / 6.f) * phip1 + (11.f / 6.f) * phi;
+  float t0 = phi - 2.f * phim1 + phim2, s0 = 3.f * phi - 4.f * phim1 + phim2;
+  float t1 = phip1 - 2.f * phi + phim1, s1 = phip1 - phim1;
+  float t2 = phip2 - 2.f * phip1 + phi, s2 = phip2 - 4.f * phip1 + 3.f * phi;
+  float b0 = (13.f / 12.f) * t0 * t0 + .25f * s0 * s0 + 1e-6f;
+  float b1 = (13.f / 12.f) * t1 * t1 + .25f * s1 * s1 + 1e-6f;
+  float b2 = (13.f / 12.f) * t2 * t2 + .25f * s2 * s2 + 1e-6f;
+  float a0 = .3f * frcp(b0 * b0);
+  float a1 = .6f * frcp(b1 * b1);
+  float a2 = .1f * frcp(b2 * b2);
+  return (a0 * p0 + a1 * p1 + a2 * p2) * frcp(a0 + a1 + a2);
+}
+
+// q[IDN] full density (geopotential for shallow water), q[IVX..IVZ]
+// velocities, q[IPR] pressure perturbation p'; P the reference pressure.
+EXO_HD void lmars_pert(int ivx, float gamma, const float *wl, const float *wr, float P,
+                       float *fl) {
+  int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  float kappa = 1.f / (gamma - 1.f);
+  float pl = P + wl[IPR], pr = P + wr[IPR];
+  float hl = pl * frcp(wl[IDN]) * (kappa + 1.f) +
+             0.5f * (wl[ivx] * wl[ivx] + wl[ivy] * wl[ivy] + wl[ivz] * wl[ivz]);
+  float hr = pr * frcp(wr[IDN]) * (kappa + 1.f) +
+             0.5f * (wr[ivx] * wr[ivx] + wr[ivy] * wr[ivy] + wr[ivz] * wr[ivz]);
+  float rhobar = 0.5f * (wl[IDN] + wr[IDN]);
+  float cbar = sqrtf(0.5f * (1.f + 1.f / kappa) * (pl + pr) * frcp(rhobar));
+  float rc = rhobar * cbar;
+  float pbar = 0.5f * (wl[IPR] + wr[IPR]) + 0.5f * rc * (wl[ivx] - wr[ivx]);
+  float ubar = 0.5f * (wl[ivx] + wr[ivx]) + 0.5f * frcp(rc) * (wl[IPR] - wr[IPR]);
+  const float *w = ubar > 0.f ? wl : wr;
+  float h = ubar > 0.f ? hl : hr;
+  float m = ubar * w[IDN];
+  fl[IDN] = m;
+  fl[ivx] = m * w[ivx] + pbar;
+  fl[ivy] = m * w[ivy];
+  fl[ivz] = m * w[ivz];
+  fl[IEN] = m * h;
+}
+
+EXO_HD void hllc_pert(int ivx, float gamma, const float *wli, const float *wri, float P,
+                      float *fl, float cosc) {
+  int ivy = IVX + ((ivx - IVX) + 1) % 3;
+  int ivz = IVX + ((ivx - IVX) + 2) % 3;
+  float igm1 = 1.f / (gamma - 1.f);
+  float dl = wli[IDN], ul = wli[ivx], vl = wli[ivy], zl = wli[ivz], ql_ = wli[IPR];
+  float dr = wri[IDN], ur = wri[ivx], vr = wri[ivy], zr = wri[ivz], qr_ = wri[IPR];
+  float pl = P + ql_, pr = P + qr_;
+  float cl = sqrtf(gamma * pl * frcp(dl)), cr = sqrtf(gamma * pr * frcp(dr));
+  float KE_l, KE_r;
+  if (ivx == IVX) {
+    float vy = vl + zl * cosc, vz = zl + vl * cosc;
+    KE_l = 0.5f * dl * (ul * ul + vl * vy + zl * vz);
+    vy = vr + zr * cosc;
+    vz = zr + vr * cosc;
+    KE_r = 0.5f * dr * (ur * ur + vr * vy + zr * vz);
+  } else {
+    KE_l = 0.5f * dl * (ul * ul + vl * vl + zl * zl);
+    KE_r = 0.5f * dr * (ur * ur + vr * vr + zr * zr);
+  }
+  float el = pl * igm1 + KE_l, er = pr * igm1 + KE_r;
+  float rhoa = .5f * (dl + dr), ca = .5f * (cl + cr);
+  float pmid = .5f * (ql_ + qr_ + (ul - ur) * rhoa * ca);  // p_mid - P
+  float c1 = (gamma + 1.f) / (2.f * gamma);
+  float qL = (pmid <= ql_) ? 1.f : sqrtf(1.f + c1 * (pmid - ql_) * frcp(pl));
+  float qR = (pmid <= qr_) ? 1.f : sqrtf(1.f + c1 * (pmid - qr_) * frcp(pr));
+  float al = ul - cl * qL, ar = ur + cr * qR;
+  float bp = ar > 0.f ? ar : 1e-20f;
+  float bm = al < 0.f ? al : -1e-20f;
+  float vxl = ul - al, vxr = ur - ar;
+  float tl = ql_ + vxl * dl * ul;  // t_l - P
+  float tr = qr_ + vxr * dr * ur;
+  float ml = dl * vxl, mr = -(dr * vxr);
+  float im = frcp(ml + mr);
+  float am = (tl - tr) * im;
+  float cp = (ml * tr + mr * tl) * im;  // contact pressure - P
+  cp = cp > -P ? cp : -P;
+  vxl = ul - bm;
+  vxr = ur - bp;
+  float fLd = dl * vxl, fRd = dr * vxr;
+  float fLx = dl * ul * vxl + ql_, fRx = dr * ur * vxr + qr_;
+  float fLy = dl * vl * vxl, fRy = dr * vr * vxr;
+  float fLz = dl * zl * vxl, fRz = dr * zr * vxr;
+  float fLe = el * vxl + pl * ul, fRe = er * vxr + pr * ur;
+  float sl, sr, sm;
+  if (am >= 0.f) {
+    float i = frcp(am - bm);
+    sl = am * i;
+    sr = 0.f;
+    sm = -bm * i;
+  } else {
+    float i = frcp(bp - am);
+    sl = 0.f;
+    sr = -am * i;
+    sm = bp * i;
+  }
+  fl[IDN] = sl * fLd + sr * fRd;
+  fl[ivx] = sl * fLx + sr * fRx + sm * cp;
+  fl[ivy] = sl * fLy + sr * fRy;
+  fl[ivz] = sl * fLz + sr * fRz;
+  fl[IEN] = sl * fLe + sr * fRe + sm * (P + cp) * am;
+}
+
+// shallow water: q[IDN] holds h' about href, H = href
+EXO_HD void roe_sw_pert(int ivx, const float *wl, const float *wr, float H, float *fl) {
+  int ivy = ivx == IVY ? IVZ : IVY;
+  float hl = H + wl[IDN], hr = H + wr[IDN];
+  float sl = sqrtf(hl), sr = sqrtf(hr);
+  float isd = frcp(sl + sr);
+  float ubar = (wl[ivx] * sl + wr[ivx] * sr) * isd;
+  float vbar = (wl[ivy] * sl + wr[ivy] * sr) * isd;
+  float cbar = sqrtf(0.5f * (hl + hr));
+  float delh = wr[IDN] - wl[IDN];
+  float delu = wr[ivx] - wl[ivx];
+  float delv = wr[ivy] - wl[ivy];
+  float hbar = sqrtf(hl * hr);
+  float ic = frcp(cbar);
+  float a1 = 0.5f * (cbar * delh - hbar * delu) * ic;
+  float a2 = hbar * delv;
+  float a3 = 0.5f * (cbar * delh + hbar * delu) * ic;
+  float s0 = fabsf(ubar - cbar), s1 = fabsf(ubar), s2 = fabsf(ubar + cbar);
+  float fd = 0.5f * (hl * wl[ivx] + hr * wr[ivx]);
+  // 0.25 (hl^2 + hr^2) - H^2 / 2 = H (h'l + h'r) / 2 + (h'l^2 + h'r^2) / 4
+  float fx = 0.5f * (hl * wl[ivx] * wl[ivx] + hr * wr[ivx] * wr[ivx]) +
+             0.5f * H * (wl[IDN] + wr[IDN]) + 0.25f * (wl[IDN] * wl[IDN] + wr[IDN] * wr[IDN]);
+  float fy = 0.5f * (hl * wl[ivx] * wl[ivy] + hr * wr[ivx] * wr[ivy]);
+  fd -= 0.5f * (s0 * a1 + s2 * a3);
+  fx -= 0.5f * (s0 * a1 * (ubar - cbar) + s2 * a3 * (ubar + cbar));
+  fy -= 0.5f * (s0 * a1 * vbar + s1 * a2 + s2 * a3 * vbar);
+  fl[IDN] = fd;
+  fl[IVX] = 0.f;
+  fl[ivx] = fx;
+  fl[ivy] = fy;
+}
+
+EXO_HD void lmars_sw_pert(int ivx, const float *wl, const float *wr, float H, float *fl) {
+  int ivy = ivx == IVY ? IVZ : IVY;
+  float hl = H + wl[IDN], hr = H + wr[IDN];
+  float cbar = sqrtf(0.5f * (hl + hr));
+  float hb = 0.5f * (wl[IDN] + wr[IDN]) + 0.5f * cbar * (wl[ivx] - wr[ivx]);  // hbar - H
+  float ubar = 0.5f * (wl[ivx] + wr[ivx]) + 0.5f * frcp(cbar) * (wl[IDN] - wr[IDN]);
+  const float *w = ubar > 0.f ? wl : wr;
+  float h = ubar > 0.f ? hl : hr;
+  fl[IDN] = ubar * h;
+  fl[ivx] = ubar * h * w[ivx] + H * hb + 0.5f * hb * hb;
+  fl[ivy] = ubar * h * w[ivy];
+  fl[IVX] = 0.f;
+}
+
+// Riemann flux without the reference-pressure part, with the cubed-sphere
+// frame maps of the FP64 path (PrimToLocal, FluxToGlobal, covariant factor).
+EXO_HD void riemann_pert(int eos, int rs, int ivx, float gamma, float *wl, float *wr, float P,
+                         float *f, float cth, float cosf, float sinf, float cosc) {
+  f[0] = f[1] = f[2] = f[3] = 0.f;
+  if (eos == EOS_IDEAL) f[4] = 0.f;
+  if (ivx != IVX && rs != RS_LMARS) {
+    local_frame<float>(ivx, cosf, sinf, wl);
+    local_frame<float>(ivx, cosf, sinf, wr);
+  }
+  if (rs == RS_ROE_SW) roe_sw_pert(ivx, wl, wr, P, f);
+  else if (rs == RS_LMARS_SW) lmars_sw_pert(ivx, wl, wr, P, f);
+  else if (rs == RS_HLLC) hllc_pert(ivx, gamma, wl, wr, P, f, cosc);
+  else lmars_pert(ivx, gamma, wl, wr, P, f);
+  if (rs != RS_LMARS) {
+    if (ivx != IVX) flux_to_global<float>(ivx, cosf, sinf, f);
+    float ty = f[IVY], tz = f[IVZ];
+    f[IVY] = ty + tz * cth;
+    f[IVZ] = tz + ty * cth;
+  }
+}
+
+}  // namespace exo

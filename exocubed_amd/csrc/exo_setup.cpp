@@ -339,6 +339,9 @@ std::string build_setup(const exo_config &cfg, Setup *s) {
   if (cfg.ntracer < 0 || cfg.ntracer > 4) return "ntracer must be 0..4";
   if (!(cfg.x1max > cfg.x1min) || !(cfg.x1min > 0)) return "bad radial range";
   if (!(cfg.cfl > 0)) return "cfl must be positive";
+  if (cfg.precision != EXO_PREC_FP64 && cfg.precision != EXO_PREC_MIXED)
+    return "unknown precision";
+  if (cfg.coriolis < EXO_COR_NONE || cfg.coriolis > EXO_COR_3D) return "unknown coriolis";
   s->cfg = cfg;
   s->N = cfg.nx;
   s->nx1 = cfg.nx1;
@@ -425,6 +428,24 @@ std::string build_setup(const exo_config &cfg, Setup *s) {
         }
         T[s->t2(T_TANX, p, k, j)] = x;
         T[s->t2(T_TANY, p, k, j)] = y;
+        // pressure-flux factors: local normal flux P -> global covariant flux
+        if (cfg.rsolver == EXO_RS_LMARS) {
+          T[s->t2(T_PRY2, p, k, j)] = 1.;
+          T[s->t2(T_PRZ2, p, k, j)] = 0.;
+          T[s->t2(T_PRY3, p, k, j)] = 0.;
+          T[s->t2(T_PRZ3, p, k, j)] = 1.;
+        } else {
+          double s2 = T[s->t2(T_SIN2, p, k, j)], c2 = T[s->t2(T_COS2, p, k, j)];
+          double s3 = T[s->t2(T_SIN3, p, k, j)], c3 = T[s->t2(T_COS3, p, k, j)];
+          double gi2 = 1.0 / (s2 * s2), gi3 = 1.0 / (s3 * s3);
+          double T22 = sqrt(gi2), T32 = -sqrt(gi2) * c2;  // FluxToGlobal2
+          double T23 = -c3 * sqrt(gi3), T33 = sqrt(gi3);  // FluxToGlobal3
+          double ch2 = T[s->t2(T_CTH2, p, k, j)], ch3 = T[s->t2(T_CTH3, p, k, j)];
+          T[s->t2(T_PRY2, p, k, j)] = T22 + T32 * ch2;
+          T[s->t2(T_PRZ2, p, k, j)] = T32 + T22 * ch2;
+          T[s->t2(T_PRY3, p, k, j)] = T23 + T33 * ch3;
+          T[s->t2(T_PRZ3, p, k, j)] = T33 + T23 * ch3;
+        }
       }
   }
   for (int p = 0; p < 6; ++p)

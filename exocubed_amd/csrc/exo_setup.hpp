@@ -31,6 +31,10 @@ enum Tab2 {
   T_TANX,      // tan(x2v(j)), Coriolis and lat-lon maps
   T_TANY,      // tan(x3v(k))
   T_CORF,      // 2 Omega sin(lat) at the cell centre (W92.cpp:59)
+  // global-frame momentum-flux factors of a unit normal pressure on an x2 /
+  // x3 face (FluxToGlobal2/3 then the covariant projection; 1/0 for lmars.cpp,
+  // which maps nothing): the mixed path adds P * factor in FP64
+  T_PRY2, T_PRZ2, T_PRY3, T_PRZ3,
   T_NTAB
 };
 
