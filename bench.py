@@ -10,6 +10,17 @@ horizontal fluxes, implicit 5x5 block-tridiagonal vertical solve
 A "step" is one full SSPRK3 step (three stages) of the whole instance; the unit
 of work is one interior cell advanced by one step (Athena's zone-cycle).
 Inputs are resident in HBM before the timed region.  One JSON line is printed.
+
+Two further measurements ride in the same line:
+  ensemble -- 256 independent 2D shallow-water instances, panel sizes drawn
+              from {24..192}, mean depths 2000-8000 m, stepped as one batch
+              (one launch per phase, autotuned persistent widths, each step a
+              replayed HIP graph), FP64 and mixed, with the compensated mass
+              drift and the batched-vs-one-at-a-time speed-up;
+  serving  -- a Poisson stream (40 req/s) of 3D forward runs (N 32..128,
+              32..96 levels, 5 variables + 2 tracers, base 150-300 K, 200
+              steps) through the continuous-batching server: p50/p99 latency
+              and aggregate cell-updates/s, FP64 and mixed.
 """
 import argparse
 import json
@@ -96,12 +107,107 @@ def cpu_baseline(cfg_fn, ex):
             "sample": f"1 SSPRK3 step of the full 6x96x96x64 workload, {sec:.1f} s wall"}
 
 
+R_EARTH = 6371000.0
+ENS_SIZES = (24, 32, 48, 64, 96, 128, 192)
+
+
+def ensemble(ex, precision, steps, n_inst=256, keep=16):
+    """256 shallow-water instances of uneven size stepped as one batch."""
+    rng = np.random.default_rng(SEED)
+    Ns = rng.choice(ENS_SIZES, n_inst)
+    h0 = rng.uniform(2000.0, 8000.0, n_inst)
+    members = []
+    for q in range(n_inst):
+        cfg = ex.make_config(nx=int(Ns[q]), nx1=1, eos=ex.EOS_SHALLOW_YZ, rsolver=ex.RS_ROE_SW,
+                             coriolis=ex.COR_FORCING2, cfl=0.4, x1min=R_EARTH,
+                             x1max=R_EARTH + 1.0, omega=7.292e-5, precision=precision)
+        u0, _ = ex.init_state(cfg, 0, SEED + q, (float(h0[q]), 50.0, 9.81))
+        g = ex.Solver(cfg)
+        g.upload(u0)
+        members.append(g)
+    cells = int(sum(6 * int(n) * int(n) for n in Ns))
+    # one at a time (a launch sequence per instance) for a few steps
+    t0 = time.perf_counter()
+    for g in members:
+        g.step(2)
+    for g in members:
+        g.sync()
+    t_alone = time.perf_counter() - t0
+    b = ex.Batch(members, graph=True)
+    caps = b.tune(reps=2)
+    b.step(2)  # warm-up: graph capture
+    m0 = b.sums()[:, 0]
+    t0 = time.perf_counter()
+    b.step(steps)
+    t1 = time.perf_counter()
+    m1 = b.sums()[:, 0]
+    b.profile_begin()
+    b.step(1)
+    prof = b.profile_end()
+    states = [members[q].download()[0] for q in range(keep)]
+    b.close()
+    for g in members:
+        g.close()
+    return {
+        "instances": n_inst, "cells": cells, "steps": steps,
+        "value": cells * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "one_at_a_time_value": cells * 2 / t_alone,
+        "mass_rel_drift_max": float(np.max(np.abs(m1 - m0) / np.abs(m0))),
+        "tuned_widths": {k: v for k, v in caps.items() if v},
+        "kernels_ms_per_step": {k: v[0] for k, v in prof.items() if v[1]},
+    }, states
+
+
+def serving(ex, precision, nreq, keep=2):
+    """Poisson request stream through the continuous-batching server."""
+    rng = np.random.default_rng(SEED + 3)
+    arrivals = np.cumsum(rng.exponential(1.0 / 40.0, nreq))
+    reqs = []
+    for q in range(nreq):
+        N = int(rng.integers(16, 65)) * 2
+        nz = int(rng.integers(32, 97))
+        T0 = float(rng.uniform(150.0, 300.0))
+        cfg = ex.make_config(nx=N, nx1=nz, eos=ex.EOS_IDEAL, rsolver=ex.RS_LMARS,
+                             implicit_flag=9, ntracer=2, coriolis=ex.COR_NONE, cfl_x1=1,
+                             x1min=R_EARTH, x1max=R_EARTH + 30000.0, gamma=1.4, grav=-9.81,
+                             omega=7.292e-5, cfl=0.4, precision=precision)
+        reqs.append((cfg, (T0, 1.0e5, 287.0, 0.01, 5.0, 3.0e-3, 0.5)))
+    srv = ex.Server(max_active=16, chunk_steps=20, graph=True, prep_threads=4)
+    ids = []
+    t0 = time.perf_counter()
+    for q, (cfg, par) in enumerate(reqs):
+        dt = t0 + arrivals[q] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(cfg, 200, kind=1, seed=SEED + 1000 + q, params=par,
+                              keep_state=q < keep))
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r[2]["latency_s"] for r in res])
+    return {
+        "requests": nreq, "steps_each": 200, "offered_req_per_s": 40.0,
+        "cells_mean": float(np.mean([r[2]["cells"] for r in res])),
+        "value": st["cell_updates"] / st["span_s"], "unit": "cell-updates/s",
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "queue_p50_s": float(np.percentile([r[2]["queue_s"] for r in res], 50)),
+        "peak_active": int(st["peak_active"]), "batches_formed": int(st["batches"]),
+        "peak_bytes": int(st["peak_bytes"]),
+        "mass_rel_drift_max": float(max(abs(r[2]["mass1"] - r[2]["mass0"]) / abs(r[2]["mass0"])
+                                        for r in res)),
+    }, [r[0] for r in res[:keep]]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--only-headline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -136,6 +242,22 @@ def main():
     prof64 = {k: v for k, v in r64["prof"].items() if v[1] > 0}
     kernels64 = {k: {"ms_per_launch": v[0] / v[1], "launches": v[1]} for k, v in prof64.items()}
     cpu = None if args.no_cpu else cpu_baseline(workload_config, ex)
+    ens = srv = None
+    if not args.only_headline:
+        e64, s64 = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)
+        emx, smx = ensemble(ex, ex.PREC_MIXED, args.ensemble_steps)
+        l2e = max(float(rel_l2(a, b)[[0, 2, 3]].max()) for a, b in zip(smx, s64))
+        ens = {"fp64": e64, "mixed": emx, "mixed_over_fp64": emx["value"] / e64["value"],
+               "mixed_rel_l2_vs_fp64_max": l2e,
+               "workload": "256 shallow-water instances, N in {24..192}, 1 layer, WENO5+Roe, "
+                           "Coriolis, SSPRK3 CFL 0.4"}
+        v64, r64s = serving(ex, ex.PREC_FP64, args.requests)
+        vmx, rmxs = serving(ex, ex.PREC_MIXED, args.requests)
+        l2s = max(float(rel_l2(a, b).max()) for a, b in zip(rmxs, r64s))
+        srv = {"fp64": v64, "mixed": vmx, "mixed_over_fp64": vmx["value"] / v64["value"],
+               "mixed_rel_l2_vs_fp64_max": l2s,
+               "workload": "Poisson 40 req/s, 3D ideal gas + 2 tracers, N 32..128, 32..96 "
+                           "levels, WENO5+LMARS, implicit 5x5, 200 steps each"}
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
@@ -160,6 +282,8 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "ensemble": ens,
+        "serving": srv,
     }
     print(json.dumps(out))
 

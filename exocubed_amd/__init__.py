@@ -19,6 +19,12 @@ EOS_SHALLOW_YZ, EOS_IDEAL = 0, 1
 RS_ROE_SW, RS_LMARS_SW, RS_HLLC, RS_LMARS = 0, 1, 2, 3
 COR_NONE, COR_FORCING2, COR_3D = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
+BATCH_GRAPH = 1
+KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "update",
+           "implicit", "dt")
+# launch kinds of the persistent-width tables (exo_tune / exo_batch_tune)
+LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
+                "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one")
 
 
 class Config(C.Structure):
@@ -86,6 +92,29 @@ def lib():
                                          C.c_int, _DP, C.c_void_p]),
         "exo_describe": (C.c_int, [C.POINTER(Config), C.c_int, C.c_void_p, C.c_size_t,
                                    C.POINTER(C.c_size_t)]),
+        "exo_conserved_sums": (C.c_int, [H, _DP]),
+        "exo_tune": (C.c_int, [H, C.c_int, C.c_void_p, C.c_int]),
+        "exo_batch_create": (C.c_int, [C.POINTER(H), C.c_int, C.c_int, C.POINTER(H)]),
+        "exo_batch_destroy": (C.c_int, [H]),
+        "exo_batch_last_error": (C.c_char_p, [H]),
+        "exo_batch_step": (C.c_int, [H, C.c_int, C.c_double]),
+        "exo_batch_sums": (C.c_int, [H, _DP]),
+        "exo_batch_tune": (C.c_int, [H, C.c_int, C.c_void_p, C.c_int]),
+        "exo_batch_set_caps": (C.c_int, [H, C.c_void_p, C.c_int]),
+        "exo_batch_profile_begin": (C.c_int, [H]),
+        "exo_batch_profile_end": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int]),
+        "exo_batch_sync": (C.c_int, [H]),
+        "exo_footprint": (C.c_size_t, [C.POINTER(Config)]),
+        "exo_server_create": (C.c_int, [C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(H)]),
+        "exo_server_submit": (C.c_int, [H, C.POINTER(Config), C.c_void_p, C.c_void_p, C.c_int,
+                                        C.c_uint64, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(C.c_uint64)]),
+        "exo_server_wait": (C.c_int, [H, C.c_uint64, C.c_double, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+        "exo_server_stats": (C.c_int, [H, C.c_void_p, C.c_int]),
+        "exo_server_last_error": (C.c_char_p, [H]),
+        "exo_server_destroy": (C.c_int, [H]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -241,3 +270,155 @@ class Solver:
 
     def sync(self):
         self._chk(self.L.exo_sync(self.h), "exo_sync")
+
+    def sums(self):
+        """Compensated FP64 totals sum(u_n * volume) per conserved variable."""
+        out = np.zeros(5)
+        self._chk(self.L.exo_conserved_sums(self.h, out), "exo_conserved_sums")
+        return out[:self.nv]
+
+    def tune(self, reps=3):
+        caps = np.zeros(len(LAUNCH_KINDS), dtype=np.int32)
+        self._chk(self.L.exo_tune(self.h, reps, caps.ctypes.data_as(C.c_void_p), len(caps)),
+                  "exo_tune")
+        return {k: int(c) for k, c in zip(LAUNCH_KINDS, caps)}
+
+
+class Batch:
+    """Several Solver instances advanced together (``exo_batch``): one launch
+    per phase over all members, optionally replayed from a captured HIP graph.
+    The members must stay alive and idle while the batch is used."""
+
+    def __init__(self, members, graph=True):
+        self.L = lib()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        rc = self.L.exo_batch_create(arr, len(self.members), BATCH_GRAPH if graph else 0,
+                                     C.byref(h))
+        if rc != 0:
+            raise ExoError(f"exo_batch_create failed ({rc})")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.exo_batch_last_error(self.h).decode()
+            raise ExoError(f"{what} failed ({rc}): {msg}")
+
+    def step(self, nsteps=1, dt=0.0):
+        self._chk(self.L.exo_batch_step(self.h, nsteps, dt), "exo_batch_step")
+
+    def sums(self):
+        out = np.zeros(len(self.members) * 5)
+        self._chk(self.L.exo_batch_sums(self.h, out), "exo_batch_sums")
+        return out.reshape(len(self.members), 5)
+
+    def tune(self, reps=3):
+        caps = np.zeros(len(LAUNCH_KINDS), dtype=np.int32)
+        self._chk(self.L.exo_batch_tune(self.h, reps, caps.ctypes.data_as(C.c_void_p),
+                                        len(caps)), "exo_batch_tune")
+        return {k: int(c) for k, c in zip(LAUNCH_KINDS, caps)}
+
+    def set_caps(self, caps):
+        arr = np.array([caps.get(k, 0) for k in LAUNCH_KINDS], dtype=np.int32)
+        self._chk(self.L.exo_batch_set_caps(self.h, arr.ctypes.data_as(C.c_void_p), len(arr)),
+                  "exo_batch_set_caps")
+
+    def profile_begin(self):
+        self._chk(self.L.exo_batch_profile_begin(self.h), "exo_batch_profile_begin")
+
+    def profile_end(self):
+        ms = np.zeros(len(KERNELS))
+        cnt = np.zeros(len(KERNELS), dtype=np.int32)
+        self._chk(self.L.exo_batch_profile_end(self.h, ms.ctypes.data_as(C.c_void_p),
+                                               cnt.ctypes.data_as(C.c_void_p), len(ms)),
+                  "exo_batch_profile_end")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNELS)}
+
+    def sync(self):
+        self._chk(self.L.exo_batch_sync(self.h), "exo_batch_sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.exo_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(cfg: Config) -> int:
+    """Device bytes an instance of this configuration allocates (host only)."""
+    return int(lib().exo_footprint(C.byref(cfg)))
+
+
+SERVER_INFO = ("latency_s", "queue_s", "run_s", "dt_last", "cells", "steps", "mass0", "mass1",
+               "total0", "total1")
+SERVER_STATS = ("completed", "cell_updates", "span_s", "busy_s", "batches", "chunks",
+                "peak_active", "peak_bytes", "budget_bytes")
+
+
+class Server:
+    """Continuous-batching request loop of the library (``exo_server``)."""
+
+    def __init__(self, mem_budget=0, max_active=64, chunk_steps=20, graph=True, prep_threads=4):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.exo_server_create(int(mem_budget), max_active, chunk_steps,
+                                      BATCH_GRAPH if graph else 0, prep_threads, C.byref(h))
+        if rc != 0:
+            raise ExoError(f"exo_server_create failed ({rc})")
+        self.h = h
+        self.cfgs = {}
+
+    def _err(self, what, rc):
+        msg = self.L.exo_server_last_error(self.h).decode()
+        return ExoError(f"{what} failed ({rc}): {msg}")
+
+    def submit(self, cfg, nsteps, u0=None, s0=None, kind=-1, seed=0, params=(),
+               keep_state=True):
+        """Queue a forward run; returns its id."""
+        rid = C.c_uint64()
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        u0 = None if u0 is None else np.ascontiguousarray(u0, dtype=np.float64)
+        s0 = None if s0 is None else np.ascontiguousarray(s0, dtype=np.float64)
+        rc = self.L.exo_server_submit(self.h, C.byref(cfg), _ptr(u0), _ptr(s0), kind, seed,
+                                      p.ctypes.data_as(C.c_void_p) if len(p) else None, len(p),
+                                      nsteps, 1 if keep_state else 0, C.byref(rid))
+        if rc != 0:
+            raise self._err("exo_server_submit", rc)
+        self.cfgs[rid.value] = (Config(**cfg.as_dict()), keep_state)
+        return rid.value
+
+    def wait(self, rid, timeout=0.0):
+        """Final state (u, s or None, None when not kept) and the info dict."""
+        cfg, keep = self.cfgs.pop(rid)
+        nv = 4 if cfg.eos == EOS_SHALLOW_YZ else 5
+        shape = (6, nv, cfg.nx + 6, cfg.nx + 6, cfg.nx1 + 6)
+        u = np.zeros(shape) if keep else None
+        s = np.zeros((6, cfg.ntracer) + shape[2:]) if keep and cfg.ntracer else None
+        info = np.zeros(len(SERVER_INFO))
+        rc = self.L.exo_server_wait(self.h, rid, timeout, _ptr(u), _ptr(s),
+                                    info.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise self._err("exo_server_wait", rc)
+        return u, s, dict(zip(SERVER_INFO, info.tolist()))
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STATS))
+        self.L.exo_server_stats(self.h, out.ctypes.data_as(C.c_void_p), len(out))
+        return dict(zip(SERVER_STATS, out.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.exo_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

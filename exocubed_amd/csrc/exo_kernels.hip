@@ -83,17 +83,60 @@ struct State {
   double *psf;                   // FP64 hydrostatic x1 face pressure [p][k][i][j]
   double *lref;                  // per level: [0,nc1) split pressure P, [nc1,2nc1) the
                                  // reconstructed variable's reference, [2nc1,3nc1) rho_ref
+  double *part;                  // per-block compensated partial sums [block][5][2]
 };
 
 constexpr int BLK = 256;
+
+// One instance as the kernels see it, and the work list of one launch.  The
+// blocks of a launch are split over the instances by a prefix table (an
+// instance that a kernel does not apply to gets no blocks); a launch with
+// fewer workgroups than blocks loops over the rest (a persistent launch), so
+// many small instances of different sizes run in one launch per phase.
+struct Inst {
+  Grid g;
+  State st;
+  const GhostDesc *gd;
+  const EdgeDesc *ed;
+  int ngd, ned;
+};
+
+struct Work {
+  const Inst *inst;    // [ninst]
+  const int *bstart;   // [ninst + 1] first block of each instance
+  int ninst, nblk;
+};
+
+// the instance owning block b: the last one whose first block is <= b
+__device__ __forceinline__ int find_inst(const Work &w, int b) {
+  int lo = 0, hi = w.ninst - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (w.bstart[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// for each block of the work list this workgroup owns: g, st and the block's
+// index lb within its instance
+#define WORK_LOOP(w)                                                     \
+  for (int b_ = blockIdx.x; b_ < (w).nblk; b_ += gridDim.x)
+#define WORK_INST(w)                                                     \
+  int ii_ = find_inst((w), b_);                                          \
+  const Grid g = (w).inst[ii_].g;                                        \
+  const State st = (w).inst[ii_].st;                                     \
+  int lb = b_ - (w).bstart[ii_]
 
 // ---------------------------------------------------------------------------
 // EquationOfState::ConservedToPrimitive (shallow_yz_hydro.cpp:42-67,
 // ideal_moist_hydro.cpp:77-159) on interior and face-ghost columns, then the
 // reflecting x1 boundary (v1 mirrored).  One thread per (panel, k, j) column.
-__global__ void k_prim(Grid g, State st, double dfloor, double sfloor) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int k = blockIdx.y, p = blockIdx.z;
+__device__ __forceinline__ void prim_body(const Grid &g, const State &st, int lb, double dfloor,
+                                          double sfloor) {
+  int nbx = (g.nc2 + 63) / 64;
+  int j = (lb % nbx) * 64 + threadIdx.x;
+  int k = (lb / nbx) % g.nc3, p = lb / (nbx * g.nc3);
   if (j >= g.nc2) return;
   bool kin = k >= g.ks && k <= g.ke, jin = j >= g.js && j <= g.je;
   if (!kin && !jin) return;  // corner ghosts are never read
@@ -151,13 +194,20 @@ __global__ void k_prim(Grid g, State st, double dfloor, double sfloor) {
     }
 }
 
+__global__ void __launch_bounds__(64) k_prim(Work w, double dfloor, double sfloor) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    prim_body(g, st, lb, dfloor, sfloor);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Panel-edge ghost remap (CubedSphereUtility::PackData with InteprolateX2/X3,
 // cubed_sphere_utility.cpp:92-617): one thread per (ghost cell, i); the
 // descriptor carries the interpolation weights and vector-map points.
-__global__ void k_ghost(Grid g, const GhostDesc *__restrict__ gd, int ndesc, double *arr,
-                        int nvar, int vectors) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ghost_body(const Grid &g, int lb, const GhostDesc *__restrict__ gd,
+                                           int ndesc, double *arr, int nvar, int vectors) {
+  int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= ndesc * ni) return;
   int di = t / ni, i = g.is + t % ni;
@@ -200,18 +250,28 @@ __global__ void k_ghost(Grid g, const GhostDesc *__restrict__ gd, int ndesc, dou
   }
 }
 
+// which = 0: conserved variables (vectors remapped), 1: passive scalars
+__global__ void __launch_bounds__(BLK) k_ghost(Work w, int which) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    const Inst &I = w.inst[ii_];
+    if (which == 0) ghost_body(g, lb, I.gd, I.ngd, st.u, g.nv, 1);
+    else ghost_body(g, lb, I.gd, I.ngd, st.s, g.nt, 0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // x1 (radial) fluxes, one thread per interior column: the hydrostatic
 // decomposition of Decomposition::ChangeToPerturbation / RestoreFromPerturbation
 // (change_to_perturbation.cpp:39-190) around Weno5X1, then the x1 Riemann
 // solve (patches/19.calculate_fluxes.patch:77-106).  The face pressure psf
 // lives in LDS, [face][lane].
-__global__ void __launch_bounds__(64) k_flux_x1(Grid g, State st) {
-  extern __shared__ double lds[];
+__device__ __forceinline__ void flux_x1_body(const Grid &g, const State &st, int lb, double *lds) {
   int lane = threadIdx.x;
-  int col = blockIdx.x * 64 + lane;
-  int p = blockIdx.y;
   int N = g.N;
+  int nbx = (N * N + 63) / 64;
+  int col = (lb % nbx) * 64 + lane;
+  int p = lb / nbx;
   bool act = col < N * N;
   int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
   double *psf = lds;  // (nc1 + 1) x 64
@@ -298,13 +358,22 @@ __global__ void __launch_bounds__(64) k_flux_x1(Grid g, State st) {
 #undef PSF
 }
 
+__global__ void __launch_bounds__(64) k_flux_x1(Work w) {
+  extern __shared__ double lds[];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    flux_x1_body(g, st, lb, lds);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Panel-edge L/R exchange (CubedSphere::sendNeighborBlocks, cubed_sphere.cpp:
 // 326-576): the interior-side WENO state of each edge face, its IVY / IVZ
 // mapped through lat-lon into the neighbour's frame, stored in the
 // neighbour's ghost-side slot [n][panel][side][a][i].
-__global__ void k_edge_states(Grid g, State st, const EdgeDesc *__restrict__ ed, int nedge) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void edge_body(const Grid &g, const State &st, int lb,
+                                          const EdgeDesc *__restrict__ ed, int nedge) {
+  int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= nedge * ni) return;
   int e = t / ni, i = g.is + t % ni;
@@ -345,12 +414,19 @@ __global__ void k_edge_states(Grid g, State st, const EdgeDesc *__restrict__ ed,
     if (n < g.nv) st.edge[n * vs + base] = q[n];
 }
 
+__global__ void __launch_bounds__(BLK) k_edge_states(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    edge_body(g, st, lb, w.inst[ii_].ed, w.inst[ii_].ned);
+  }
+}
+
 // x2 faces (dir = 1) or x3 faces (dir = 2): WENO5 L/R of the face, the panel
 // edge's ghost side from the exchanged states, and Hydro::RiemannSolver with
 // the cubed-sphere frame maps.  One thread per (panel, k, i, j) face.
 template <int dir>
-__global__ void k_flux_h(Grid g, State st) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flux_h_body(const Grid &g, const State &st, int lb) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   int na = N + 1;  // faces along the normal axis
   size_t total = (size_t)6 * N * ni * na;
@@ -438,6 +514,14 @@ __global__ void k_flux_h(Grid g, State st) {
   }
 }
 
+template <int dir>
+__global__ void __launch_bounds__(BLK) k_flux_h(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    flux_h_body<dir>(g, st, lb);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Stage update of one interior cell: SSPRK3 weighted average (u1, u),
 // Hydro::AddFluxDivergence, GnomonicEquiangle::AddCoordTermsDivergence
@@ -451,8 +535,8 @@ struct StageW {
 };
 
 template <bool MIX>
-__global__ void k_update(Grid g, State st, StageW sw) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void update_body(const Grid &g, const State &st, int lb, StageW sw) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
   if (t >= total) return;
@@ -596,14 +680,24 @@ __global__ void k_update(Grid g, State st, StageW sw) {
   }
 }
 
+template <bool MIX>
+__global__ void __launch_bounds__(BLK) k_update(Work w, StageW sw) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    update_body<MIX>(g, st, lb, sw);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ImplicitSolver::FullCorrection on an x1 column (full_correction.cpp:32-165,
 // forward_backward.hpp:15-126), one thread per column; the forward sweep keeps
 // the previous block row in registers and spills a[i] and delta[i] to the
 // coefficient array for the backward substitution.  Ends with u += du.
-__global__ void __launch_bounds__(64) k_implicit(Grid g, State st, double beta) {
-  int col = blockIdx.x * 64 + threadIdx.x;
-  int p = blockIdx.y, N = g.N;
+__device__ __forceinline__ void implicit_body(const Grid &g, const State &st, int lb, double beta) {
+  int N = g.N;
+  int nbx = (N * N + 63) / 64;
+  int col = (lb % nbx) * 64 + threadIdx.x;
+  int p = lb / nbx;
   if (col >= N * N) return;
   int k = g.ks + col / N, j = g.js + col % N;
   int is = g.is, ie = g.ie;
@@ -723,6 +817,13 @@ __global__ void __launch_bounds__(64) k_implicit(Grid g, State st, double beta) 
       st.u[cc] += d.a[n];
     }
     dnext = d;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_implicit(Work w, double beta) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    implicit_body(g, st, lb, beta);
   }
 }
 
@@ -870,22 +971,21 @@ __device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) 
   }
 }
 
-__global__ void __launch_bounds__(64) k_implicit_grp(Grid g, State st, double beta) {
-  __shared__ double lds[IGPW * ILDS];
+__device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st, int lb,
+                                                  double beta, double *lds, double *lap) {
   int lane = threadIdx.x;
   int grp = lane / IG, r = lane % IG;
   bool lane_ok = grp < IGPW;
   if (!lane_ok) { grp = 0; r = 0; }
   int N = g.N;
-  int col = blockIdx.x * IGPW + grp;
+  int nbx = (N * N + IGPW - 1) / IGPW;
+  int col = (lb % nbx) * IGPW + grp;
   bool act = lane_ok && col < N * N;
-  int p = blockIdx.y;
+  int p = lb / nbx;
   int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
   double *L = lds + grp * ILDS;
   double *Lri = L, *LT = L + 25, *LX = L + 50, *LC = L + 75, *Ltv = L + 100, *Ldp = L + 105;
-  // Aprev rows live in LT between levels is re-used; keep them separately:
-  __shared__ double lap[IGPW * 25];
-  double *LA = lap + grp * 25;
+  double *LA = lap + grp * 25;  // a^-1 c of the previous level
   int is = g.is, ie = g.ie;
   double dt = beta * st.dtv[0];
   double gamma = g.gamma, grav = g.grav;
@@ -1038,9 +1138,19 @@ __global__ void __launch_bounds__(64) k_implicit_grp(Grid g, State st, double be
   }
 }
 
+__global__ void __launch_bounds__(64) k_implicit_grp(Work w, double beta) {
+  __shared__ double lds[IGPW * ILDS];
+  __shared__ double lap[IGPW * 25];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    implicit_grp_body(g, st, lb, beta, lds, lap);
+    __syncthreads();
+  }
+}
+
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
-__global__ void k_add_du(Grid g, State st) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void add_du_body(const Grid &g, const State &st, int lb) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
   if (t >= total) return;
@@ -1055,14 +1165,29 @@ __global__ void k_add_du(Grid g, State st) {
   }
 }
 
+__global__ void __launch_bounds__(BLK) k_add_du(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    add_du_body(g, st, lb);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Hydro::NewBlockTimeStep (patches/21.new_blockdt.patch:16-37): per-cell
 // width / signal speed, block minimum, device-wide minimum on the IEEE bits
 // (positive doubles order like unsigned integers).
-__global__ void k_dt_reset(State st) { *st.dtbits = 0x7FF0000000000000ull; }
+// one block per instance (work list of one block each), thread 0 acts
+__global__ void k_dt_reset(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    (void)g;
+    (void)lb;
+    if (threadIdx.x == 0) *st.dtbits = 0x7FF0000000000000ull;
+  }
+}
 
-__global__ void k_dt(Grid g, State st) {
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void dt_body(const Grid &g, const State &st, int lb, double *wmin) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
   double m = HUGE_VAL;
@@ -1092,7 +1217,6 @@ __global__ void k_dt(Grid g, State st) {
     double o = __shfl_xor(m, off);
     m = o < m ? o : m;
   }
-  __shared__ double wmin[BLK / 64];
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) wmin[wv] = m;
   __syncthreads();
@@ -1103,19 +1227,140 @@ __global__ void k_dt(Grid g, State st) {
   }
 }
 
-// dt = min(cfl * min_cells, 2 dt_prev) (Mesh::NewTimeStep growth limit)
-__global__ void k_dt_finish(State st, double cfl, double dt_fixed) {
-  if (dt_fixed > 0.0) {
-    st.dtv[0] = dt_fixed;
-    return;
+__global__ void __launch_bounds__(BLK) k_dt(Work w) {
+  __shared__ double wmin[BLK / 64];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    dt_body(g, st, lb, wmin);
+    __syncthreads();  // wmin is reused by the next block of the loop
   }
-  double d = __longlong_as_double((long long)*st.dtbits) * cfl;
-  double dt = 2.0 * st.dtv[1];
-  if (d < dt) dt = d;
-  st.dtv[0] = dt;
-  st.dtv[1] = dt;
 }
 
+// dt = min(cfl * min_cells, 2 dt_prev) (Mesh::NewTimeStep growth limit)
+__global__ void k_dt_finish(Work w, double dt_fixed) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    (void)lb;
+    if (threadIdx.x != 0) continue;
+    if (dt_fixed > 0.0) {
+      st.dtv[0] = dt_fixed;
+      continue;
+    }
+    double d = __longlong_as_double((long long)*st.dtbits) * g.cfl;
+    double dt = 2.0 * st.dtv[1];
+    if (d < dt) dt = d;
+    st.dtv[0] = dt;
+    st.dtv[1] = dt;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Conserved totals sum_cells u_n * volume in FP64 with compensated (TwoSum /
+// Neumaier) addition: per-block (sum, error) pairs in a fixed tree order, then
+// one block per instance adds its partials in a fixed order, so the totals are
+// deterministic and accurate to about one rounding of the result.
+struct Pair {
+  double s, c;
+};
+
+__device__ __forceinline__ Pair padd(Pair a, Pair b) {
+  double s = a.s + b.s;
+  double bb = s - a.s;
+  double err = (a.s - (s - bb)) + (b.s - bb);  // exact error of a.s + b.s
+  return Pair{s, a.c + b.c + err};
+}
+
+__device__ __forceinline__ Pair wave_sum(Pair p) {
+  for (int off = 32; off > 0; off >>= 1) {
+    Pair o{__shfl_xor(p.s, off), __shfl_xor(p.c, off)};
+    // the lower lane of each pair adds in the same order as the upper one
+    p = (threadIdx.x & off) ? padd(o, p) : padd(p, o);
+  }
+  return p;
+}
+
+// block-wide compensated sum of 5 pairs; the result is valid in thread 0
+__device__ __forceinline__ void block_sum5(Pair *acc, double *lds) {
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    acc[n] = wave_sum(acc[n]);
+    if (lane == 0) {
+      lds[(wv * 5 + n) * 2] = acc[n].s;
+      lds[(wv * 5 + n) * 2 + 1] = acc[n].c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      Pair t{lds[n * 2], lds[n * 2 + 1]};
+      for (int q = 1; q < nw; ++q) t = padd(t, Pair{lds[(q * 5 + n) * 2], lds[(q * 5 + n) * 2 + 1]});
+      acc[n] = t;
+    }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void sums_body(const Grid &g, const State &st, int lb, double *lds) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * N * ni * N;
+  Pair acc[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) acc[n] = Pair{0., 0.};
+  if (t < total) {
+    int j = g.js + (int)(t % N);
+    size_t r = t / N;
+    int i = g.is + (int)(r % ni);
+    r /= ni;
+    int k = g.ks + (int)(r % N), p = (int)(r / N);
+    double vol = g.volume(p, k, j, i);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) acc[n].s = st.u[g.c(n, p, k, i, j)] * vol;
+  }
+  block_sum5(acc, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      st.part[((size_t)lb * 5 + n) * 2] = acc[n].s;
+      st.part[((size_t)lb * 5 + n) * 2 + 1] = acc[n].c;
+    }
+}
+
+// cell-shaped work list (one block per BLK interior cells)
+__global__ void __launch_bounds__(BLK) k_sums(Work w) {
+  __shared__ double lds[(BLK / 64) * 5 * 2];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    sums_body(g, st, lb, lds);
+  }
+}
+
+// one block per instance; cstart is the cell-shaped table (partials per
+// instance), out[instance][5] receives the rounded compensated totals
+__global__ void __launch_bounds__(BLK) k_sums_final(Work w, const int *__restrict__ cstart,
+                                                   double *out) {
+  __shared__ double lds[(BLK / 64) * 5 * 2];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    (void)g;
+    (void)lb;
+    int nb = cstart[ii_ + 1] - cstart[ii_];
+    Pair acc[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) acc[n] = Pair{0., 0.};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+      for (int n = 0; n < 5; ++n)
+        acc[n] = padd(acc[n], Pair{st.part[((size_t)b * 5 + n) * 2],
+                                   st.part[((size_t)b * 5 + n) * 2 + 1]});
+    block_sum5(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) out[(size_t)ii_ * 5 + n] = acc[n].s + acc[n].c;
+  }
+}
 
 // ===========================================================================
 // Mixed precision (EXO_PREC_MIXED): FP32 WENO5 + Riemann on perturbations
@@ -1126,13 +1371,14 @@ __global__ void k_dt_finish(State st, double cfl, double dt_fixed) {
 
 // x1 fluxes: FP64 decomposition (psf in LDS, stored for the update), FP32
 // reconstruction of the perturbation and FP32 Riemann flux.
-__global__ void __launch_bounds__(64) k_flux_x1_mix(Grid g, State st) {
+__device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st, int lb,
+                                                 double *lds) {
 #pragma clang fp contract(fast)
-  extern __shared__ double lds[];
   int lane = threadIdx.x;
-  int col = blockIdx.x * 64 + lane;
-  int p = blockIdx.y;
   int N = g.N;
+  int nbx = (N * N + 63) / 64;
+  int col = (lb % nbx) * 64 + lane;
+  int p = lb / nbx;
   if (col >= N * N) return;
   int k = g.ks + col / N, j = g.js + col % N;
   double *psf = lds;
@@ -1209,6 +1455,14 @@ __global__ void __launch_bounds__(64) k_flux_x1_mix(Grid g, State st) {
 #undef PSF
 }
 
+__global__ void __launch_bounds__(64) k_flux_x1_mix(Work w) {
+  extern __shared__ double lds[];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    flux_x1_mix_body(g, st, lb, lds);
+  }
+}
+
 // cell value as an FP32 perturbation about the level reference
 __device__ __forceinline__ float pert(const Grid &g, const State &st, int n, int p, int k,
                                       int i, int j) {
@@ -1222,10 +1476,10 @@ __device__ __forceinline__ float pert(const Grid &g, const State &st, int n, int
   return (float)v;
 }
 
-__global__ void k_edge_states_mix(Grid g, State st, const EdgeDesc *__restrict__ ed,
-                                  int nedge) {
+__device__ __forceinline__ void edge_mix_body(const Grid &g, const State &st, int lb,
+                                              const EdgeDesc *__restrict__ ed, int nedge) {
 #pragma clang fp contract(fast)
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= nedge * ni) return;
   int e = t / ni, i = g.is + t % ni;
@@ -1266,10 +1520,17 @@ __global__ void k_edge_states_mix(Grid g, State st, const EdgeDesc *__restrict__
     if (n < g.nv) st.fedge[n * vs + base] = q[n];
 }
 
+__global__ void __launch_bounds__(BLK) k_edge_states_mix(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    edge_mix_body(g, st, lb, w.inst[ii_].ed, w.inst[ii_].ned);
+  }
+}
+
 template <int dir>
-__global__ void k_flux_h_mix(Grid g, State st) {
+__device__ __forceinline__ void flux_h_mix_body(const Grid &g, const State &st, int lb) {
 #pragma clang fp contract(fast)
-  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   int na = N + 1;
   size_t total = (size_t)6 * N * ni * na;
@@ -1353,6 +1614,14 @@ __global__ void k_flux_h_mix(Grid g, State st) {
   }
 }
 
+template <int dir>
+__global__ void __launch_bounds__(BLK) k_flux_h_mix(Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    flux_h_mix_body<dir>(g, st, lb);
+  }
+}
+
 // single-face / scalar shims
 __global__ void k_riemann_face(Grid g, int dir, int p, int k, int j, const double *wl0,
                                const double *wr0, double *flx) {
@@ -1386,26 +1655,66 @@ __global__ void k_math_batch(int op, const double *x, double y, double *out, int
 }  // namespace
 
 // ===========================================================================
-// handle
+// launches: every kernel runs over a work list of instances (one for a
+// handle, many for a batch)
+namespace {
+
+// launch kinds, each with its own block-prefix table over the instances
+enum {
+  K_PRIM = 0, K_GHOST_U, K_GHOST_S, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX,
+  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_NKIND
+};
+
+// kernel ids of the profile (exo_profile_end)
+enum { KID_PRIM = 0, KID_GHOST, KID_FLUX_X1, KID_EDGE, KID_FLUX_X2, KID_FLUX_X3, KID_UPDATE,
+       KID_IMPLICIT, KID_DT, KID_COUNT };
+
+unsigned grid1(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
+
+// density and scalar mixing-ratio floors, sqrt(1024 FLT_MIN)
+const double DFLOOR = std::sqrt(1024 * (double)FLT_MIN);
+const double SFLOOR = std::sqrt(1024 * (double)FLT_MIN);
+
+struct Launcher {
+  hipStream_t stream = nullptr;
+  int ninst = 0;
+  Inst *d_inst = nullptr;
+  int *d_bstart = nullptr;  // [K_NKIND][ninst + 1]
+  int nblk[K_NKIND] = {};
+  int cap[K_NKIND] = {};    // workgroups of a persistent launch (0: one per block)
+  size_t lds_x1 = 0;        // dynamic LDS of the x1 kernels (largest instance)
+  bool per_lane = false;    // implicit solve with one column per lane
+  // live per-kernel timing with HIP events on the stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kind;
+
+  Work work(int kind) const {
+    return Work{d_inst, d_bstart + (size_t)kind * (ninst + 1), ninst, nblk[kind]};
+  }
+  unsigned grid(int kind) const {
+    int n = nblk[kind];
+    return (unsigned)(cap[kind] > 0 && cap[kind] < n ? cap[kind] : n);
+  }
+};
+
+}  // namespace
+
 struct exo_handle {
   Setup setup;
   Grid g;
   State st;
-  hipStream_t stream = nullptr;
+  Launcher L;                   // this instance alone
+  hipStream_t stream = nullptr; // == L.stream
   std::string err;
   std::vector<void *> allocs;
   GhostDesc *d_ghost = nullptr;
   EdgeDesc *d_edge = nullptr;
   double *d_x1 = nullptr, *d_tab = nullptr;
-  double dfloor, sfloor;
+  double *d_sums = nullptr;     // [5] totals of exo_conserved_sums
   bool have_state = false;
   bool mixed = false;
-  bool implicit_per_lane = false;  // EXO_IMPLICIT_PER_LANE=1: one column per lane
-  // live per-kernel timing with HIP events on the handle's stream
-  bool prof = false;
-  std::vector<hipEvent_t> ev_pool;
-  size_t ev_used = 0;
-  std::vector<int> ev_kind;  // kernel id of each (start, end) pair
 
   void *dalloc(size_t bytes) {
     void *p = nullptr;
@@ -1418,40 +1727,114 @@ struct exo_handle {
   size_t face_bytes(int nvar) const { return (size_t)nvar * g.fvstride * sizeof(double); }
 };
 
-namespace {
-
-unsigned grid1(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
-
-// kernel ids of the profile (exo_profile_end)
-enum { KID_PRIM = 0, KID_GHOST, KID_FLUX_X1, KID_EDGE, KID_FLUX_X2, KID_FLUX_X3, KID_UPDATE,
-       KID_IMPLICIT, KID_DT, KID_COUNT };
-
-hipEvent_t next_event(exo_handle *h) {
-  if (h->ev_used == h->ev_pool.size()) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    h->ev_pool.push_back(e);
-  }
-  return h->ev_pool[h->ev_used++];
-}
-
-// records an event pair around one launch when profiling is on
-struct ProfScope {
-  exo_handle *h;
-  ProfScope(exo_handle *hh, int kid) : h(hh) {
-    if (h->prof) {
-      h->ev_kind.push_back(kid);
-      HIPCHK(hipEventRecord(next_event(h), h->stream));
-    }
-  }
-  ~ProfScope() noexcept(false) {
-    if (h->prof) HIPCHK(hipEventRecord(next_event(h), h->stream));
-  }
+// A set of instances advanced together: one launch per phase over all of
+// them, optionally persistent, each step replayed from a captured HIP graph.
+struct exo_batch {
+  std::vector<exo_handle *> members;
+  Launcher L;
+  std::string err;
+  int use_graph = 1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double graph_dt_fixed = 0.0;
+  double *d_sums = nullptr;  // [members][5]
 };
 
-void init_handle(exo_handle *h) {
-  Setup &s = h->setup;
-  Grid &g = h->g;
+namespace {
+
+hipEvent_t next_event(Launcher &L) {
+  if (L.ev_used == L.ev_pool.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    L.ev_pool.push_back(e);
+  }
+  return L.ev_pool[L.ev_used++];
+}
+
+// blocks of one instance in a launch of the given kind (0: not applicable)
+int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, bool per_lane) {
+  bool ideal = g.eos == EOS_IDEAL;
+  size_t cells = (size_t)6 * g.N * g.N * g.nx1;
+  size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
+  size_t cols = (size_t)g.N * g.N;
+  switch (kind) {
+    case K_PRIM: return ((g.nc2 + 63) / 64) * g.nc3 * 6;
+    case K_GHOST_U: return (int)grid1((size_t)ngd * g.nx1);
+    case K_GHOST_S: return g.nt ? (int)grid1((size_t)ngd * g.nx1) : 0;
+    case K_X1: return ideal && !mixed ? (int)grid1(cols, 64) * 6 : 0;
+    case K_X1_MIX: return ideal && mixed ? (int)grid1(cols, 64) * 6 : 0;
+    case K_EDGE: return !mixed ? (int)grid1((size_t)ned * g.nx1) : 0;
+    case K_EDGE_MIX: return mixed ? (int)grid1((size_t)ned * g.nx1) : 0;
+    case K_FACE: return !mixed ? (int)grid1(faces) : 0;
+    case K_FACE_MIX: return mixed ? (int)grid1(faces) : 0;
+    case K_UPD: return !mixed ? (int)grid1(cells) : 0;
+    case K_UPD_MIX: return mixed ? (int)grid1(cells) : 0;
+    case K_IMPL:
+      if (!ideal || !g.implicit) return 0;
+      return (int)(per_lane ? grid1(cols, 64) : grid1(cols, IGPW)) * 6;
+    case K_ADD_DU: return ideal && !g.implicit ? (int)grid1(cells) : 0;
+    case K_CELLS: return (int)grid1(cells);
+    case K_ONE: return 1;
+  }
+  return 0;
+}
+
+// uploads the instance list and block tables of the given handles
+void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
+                    std::vector<void *> &allocs) {
+  int n = (int)hs.size();
+  L.ninst = n;
+  std::vector<Inst> inst(n);
+  std::vector<int> tab((size_t)K_NKIND * (n + 1));
+  L.lds_x1 = 0;
+  for (int i = 0; i < n; ++i) {
+    const exo_handle *h = hs[i];
+    inst[i].g = h->g;
+    inst[i].st = h->st;
+    inst[i].gd = h->d_ghost;
+    inst[i].ed = h->d_edge;
+    inst[i].ngd = (int)h->setup.ghosts.size();
+    inst[i].ned = (int)h->setup.edges.size();
+    L.lds_x1 = std::max(L.lds_x1, (size_t)(h->g.nc1 + 1) * 64 * sizeof(double));
+  }
+  for (int kind = 0; kind < K_NKIND; ++kind) {
+    long long acc = 0;
+    for (int i = 0; i < n; ++i) {
+      tab[(size_t)kind * (n + 1) + i] = (int)acc;
+      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned, L.per_lane);
+    }
+    if (acc > 0x7fffffffLL) throw std::string("work list too large for one launch");
+    tab[(size_t)kind * (n + 1) + n] = (int)acc;
+    L.nblk[kind] = (int)acc;
+  }
+  void *p;
+  HIPCHK(hipMalloc(&p, sizeof(Inst) * n));
+  allocs.push_back(p);
+  L.d_inst = (Inst *)p;
+  HIPCHK(hipMalloc(&p, sizeof(int) * tab.size()));
+  allocs.push_back(p);
+  L.d_bstart = (int *)p;
+  HIPCHK(hipMemcpyAsync(L.d_inst, inst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, L.stream));
+  HIPCHK(hipMemcpyAsync(L.d_bstart, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice,
+                        L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
+}
+
+// one launch of a work-list kernel; kid < 0: not timed
+template <typename... P, typename... A>
+void launch(Launcher &L, int kind, int kid, void (*kern)(Work, P...), unsigned block, size_t lds,
+            A... args) {
+  if (L.nblk[kind] == 0) return;
+  bool t = L.prof && kid >= 0;
+  if (t) {
+    L.ev_kind.push_back(kid);
+    HIPCHK(hipEventRecord(next_event(L), L.stream));
+  }
+  kern<<<dim3(L.grid(kind)), dim3(block), lds, L.stream>>>(L.work(kind), args...);
+  if (t) HIPCHK(hipEventRecord(next_event(L), L.stream));
+}
+
+void make_grid(const Setup &s, Grid &g) {
   g.N = s.N;
   g.nx1 = s.nx1;
   g.nc1 = s.nc1;
@@ -1476,10 +1859,58 @@ void init_handle(exo_handle *h) {
   g.fpstride = (size_t)(g.nc3 + 1) * g.fplane;
   g.fvstride = 6 * g.fpstride;
   g.t2stride = (size_t)6 * g.nc3 * g.nc2;
-  h->dfloor = sqrt(1024 * (double)FLT_MIN);
-  h->sfloor = sqrt(1024 * (double)FLT_MIN);
+}
+
+// Device bytes one instance allocates (the list init_handle allocates, each
+// rounded up to the allocation granule), for admission by memory footprint.
+size_t footprint_bytes(const Setup &s) {
+  Grid g;
+  make_grid(s, g);
+  const size_t gran = (size_t)64 << 10;
+  size_t tot = 0;
+  auto add = [&](size_t b) { tot += (b + gran - 1) / gran * gran; };
+  bool mixed = s.cfg.precision == EXO_PREC_MIXED;
+  size_t cell = g.vstride * sizeof(double), face = g.fvstride * sizeof(double);
+  size_t edge = (size_t)6 * 4 * g.N * g.nx1;
+  add(3 * (s.nc1 + 1) * sizeof(double));
+  add(s.tab2.size() * sizeof(double));
+  add(s.ghosts.size() * sizeof(GhostDesc));
+  add(s.edges.size() * sizeof(EdgeDesc));
+  for (int q = 0; q < 4; ++q) add(cell * g.nv);           // u, w, u1, du
+  if (g.nt)
+    for (int q = 0; q < 3; ++q) add(cell * g.nt);         // s, r, s1
+  if (!mixed) {
+    for (int d = 0; d < 3; ++d) add(face * g.nv);
+    if (g.nt)
+      for (int d = 0; d < 3; ++d) add(face * g.nt);
+    add(edge * g.nv * sizeof(double));
+  }
+  if (g.eos == EXO_EOS_IDEAL && g.implicit) add(cell * 30);
+  if (mixed) {
+    for (int d = 0; d < 3; ++d) {
+      add(g.fvstride * sizeof(float) * g.nv);
+      if (g.nt) add(g.fvstride * sizeof(float) * g.nt);
+    }
+    add(edge * g.nv * sizeof(float));
+    if (g.eos == EXO_EOS_IDEAL) add(face);
+    add(3 * (size_t)g.nc1 * sizeof(double));
+  }
+  add(sizeof(unsigned long long));
+  add(2 * sizeof(double));
+  add((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
+  add(5 * sizeof(double));
+  add(sizeof(Inst));
+  add(sizeof(int) * K_NKIND * 2);
+  return tot;
+}
+
+void init_handle(exo_handle *h) {
+  Setup &s = h->setup;
+  Grid &g = h->g;
+  make_grid(s, g);
 
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->L.stream = h->stream;
   // geometry
   std::vector<double> x1(3 * (s.nc1 + 1), 0.);
   for (int i = 0; i <= s.nc1; ++i) x1[i] = s.x1f[i];
@@ -1512,15 +1943,18 @@ void init_handle(exo_handle *h) {
     st.s = (double *)h->dalloc(h->cells_bytes(g.nt));
     st.r = (double *)h->dalloc(h->cells_bytes(g.nt));
     st.s1 = (double *)h->dalloc(h->cells_bytes(g.nt));
-    for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
   }
-  for (int d = 0; d < 3; ++d) st.flx[d] = (double *)h->dalloc(h->face_bytes(g.nv));
-  st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
-  if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
   h->mixed = s.cfg.precision == EXO_PREC_MIXED;
+  if (!h->mixed) {
+    for (int d = 0; d < 3; ++d) st.flx[d] = (double *)h->dalloc(h->face_bytes(g.nv));
+    if (g.nt)
+      for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
+    st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
+  }
+  if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
   {
     const char *e = getenv("EXO_IMPLICIT_PER_LANE");
-    h->implicit_per_lane = e && e[0] == '1';
+    h->L.per_lane = e && e[0] == '1';
   }
   if (h->mixed) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
@@ -1534,85 +1968,30 @@ void init_handle(exo_handle *h) {
   }
   st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
   st.dtv = (double *)h->dalloc(2 * sizeof(double));
+  st.part = (double *)h->dalloc((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
+  h->d_sums = (double *)h->dalloc(5 * sizeof(double));
   double dt0[2] = {0., DBL_MAX * 0.4};
   HIPCHK(hipMemcpyAsync(st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
+  launcher_setup(h->L, std::vector<exo_handle *>{h}, h->allocs);
 }
 
-void launch_fill_ghosts(exo_handle *h) {
-  Grid &g = h->g;
-  State &st = h->st;
-  int nd = (int)h->setup.ghosts.size();
-  {
-    ProfScope ps_(h, KID_GHOST);
-    hipLaunchKernelGGL(k_ghost, dim3(grid1((size_t)nd * g.nx1)), dim3(BLK), 0, h->stream, g,
-                       h->d_ghost, nd, st.u, g.nv, 1);
-  }
-  if (g.nt)
-    {
-      ProfScope ps_(h, KID_GHOST);
-      hipLaunchKernelGGL(k_ghost, dim3(grid1((size_t)nd * g.nx1)), dim3(BLK), 0, h->stream, g,
-                         h->d_ghost, nd, st.s, g.nt, 0);
-    }
-  {
-    ProfScope ps_(h, KID_PRIM);
-    hipLaunchKernelGGL(k_prim, dim3(grid1(g.nc2, 64), g.nc3, 6), dim3(64), 0, h->stream, g, st,
-                       h->dfloor, h->sfloor);
-  }
+void launch_fill_ghosts(Launcher &L) {
+  launch(L, K_GHOST_U, KID_GHOST, k_ghost, BLK, 0, 0);
+  launch(L, K_GHOST_S, KID_GHOST, k_ghost, BLK, 0, 1);
+  launch(L, K_PRIM, KID_PRIM, k_prim, 64, 0, DFLOOR, SFLOOR);
   HIPCHK(hipGetLastError());
 }
 
-void launch_fluxes(exo_handle *h) {
-  Grid &g = h->g;
-  State &st = h->st;
-  if (h->mixed) {
-    if (g.eos == EXO_EOS_IDEAL) {
-      size_t lds = (size_t)(g.nc1 + 1) * 64 * sizeof(double);
-      ProfScope ps_(h, KID_FLUX_X1);
-      hipLaunchKernelGGL(k_flux_x1_mix, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), lds,
-                         h->stream, g, st);
-    }
-    int ne = (int)h->setup.edges.size();
-    {
-      ProfScope ps_(h, KID_EDGE);
-      hipLaunchKernelGGL(k_edge_states_mix, dim3(grid1((size_t)ne * g.nx1)), dim3(BLK), 0,
-                         h->stream, g, st, h->d_edge, ne);
-    }
-    size_t nf = (size_t)6 * g.N * g.nx1 * (g.N + 1);
-    {
-      ProfScope ps_(h, KID_FLUX_X2);
-      hipLaunchKernelGGL(k_flux_h_mix<1>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
-    }
-    {
-      ProfScope ps_(h, KID_FLUX_X3);
-      hipLaunchKernelGGL(k_flux_h_mix<2>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
-    }
-    HIPCHK(hipGetLastError());
-    return;
-  }
-  if (g.eos == EXO_EOS_IDEAL) {
-    size_t lds = (size_t)(g.nc1 + 1) * 64 * sizeof(double);
-    {
-      ProfScope ps_(h, KID_FLUX_X1);
-      hipLaunchKernelGGL(k_flux_x1, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), lds,
-                         h->stream, g, st);
-    }
-  }
-  int ne = (int)h->setup.edges.size();
-  {
-    ProfScope ps_(h, KID_EDGE);
-    hipLaunchKernelGGL(k_edge_states, dim3(grid1((size_t)ne * g.nx1)), dim3(BLK), 0, h->stream,
-                       g, st, h->d_edge, ne);
-  }
-  size_t nf = (size_t)6 * g.N * g.nx1 * (g.N + 1);
-  {
-    ProfScope ps_(h, KID_FLUX_X2);
-    hipLaunchKernelGGL(k_flux_h<1>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
-  }
-  {
-    ProfScope ps_(h, KID_FLUX_X3);
-    hipLaunchKernelGGL(k_flux_h<2>, dim3(grid1(nf)), dim3(BLK), 0, h->stream, g, st);
-  }
+void launch_fluxes(Launcher &L) {
+  launch(L, K_X1, KID_FLUX_X1, k_flux_x1, 64, L.lds_x1);
+  launch(L, K_X1_MIX, KID_FLUX_X1, k_flux_x1_mix, 64, L.lds_x1);
+  launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
+  launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
+  launch(L, K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
+  launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
+  launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1>, BLK, 0);
+  launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2>, BLK, 0);
   HIPCHK(hipGetLastError());
 }
 
@@ -1621,63 +2000,104 @@ const double SS_G1[3] = {0.0, 0.25, 2.0 / 3.0};
 const double SS_G2[3] = {1.0, 0.75, 1.0 / 3.0};
 const double SS_BETA[3] = {1.0, 0.25, 2.0 / 3.0};
 
-void launch_update(exo_handle *h, int stage) {
-  Grid &g = h->g;
+void launch_update(Launcher &L, int stage) {
   int st0 = stage - 1;
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0};
-  size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
-  {
-    ProfScope ps_(h, KID_UPDATE);
-    if (h->mixed)
-      hipLaunchKernelGGL(k_update<true>, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
-    else
-      hipLaunchKernelGGL(k_update<false>, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st, sw);
-  }
+  launch(L, K_UPD, KID_UPDATE, k_update<false>, BLK, 0, sw);
+  launch(L, K_UPD_MIX, KID_UPDATE, k_update<true>, BLK, 0, sw);
   HIPCHK(hipGetLastError());
 }
 
-void launch_implicit(exo_handle *h, double beta) {
-  Grid &g = h->g;
-  if (g.eos != EXO_EOS_IDEAL) return;
-  if (g.implicit) {
-    {
-      ProfScope ps_(h, KID_IMPLICIT);
-      if (h->implicit_per_lane)
-        hipLaunchKernelGGL(k_implicit, dim3(grid1((size_t)g.N * g.N, 64), 6), dim3(64), 0,
-                           h->stream, g, h->st, beta);
-      else
-        hipLaunchKernelGGL(k_implicit_grp, dim3(grid1((size_t)g.N * g.N, IGPW), 6), dim3(64), 0,
-                           h->stream, g, h->st, beta);
-    }
-  } else {
-    size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
-    {
-      ProfScope ps_(h, KID_IMPLICIT);
-      hipLaunchKernelGGL(k_add_du, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st);
-    }
-  }
+void launch_implicit(Launcher &L, double beta) {
+  if (L.per_lane) launch(L, K_IMPL, KID_IMPLICIT, k_implicit, 64, 0, beta);
+  else launch(L, K_IMPL, KID_IMPLICIT, k_implicit_grp, 64, 0, beta);
+  launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
   HIPCHK(hipGetLastError());
 }
 
-void launch_dt(exo_handle *h, double dt_fixed) {
-  Grid &g = h->g;
+void launch_dt(Launcher &L, double dt_fixed) {
   if (dt_fixed <= 0.0) {
-    hipLaunchKernelGGL(k_dt_reset, dim3(1), dim3(1), 0, h->stream, h->st);
-    size_t nc = (size_t)6 * g.N * g.nx1 * g.N;
-    {
-      ProfScope ps_(h, KID_DT);
-      hipLaunchKernelGGL(k_dt, dim3(grid1(nc)), dim3(BLK), 0, h->stream, g, h->st);
-    }
+    launch(L, K_ONE, -1, k_dt_reset, 64, 0);
+    launch(L, K_CELLS, KID_DT, k_dt, BLK, 0);
   }
-  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(1), 0, h->stream, h->st, g.cfl, dt_fixed);
+  launch(L, K_ONE, -1, k_dt_finish, 64, 0, dt_fixed);
   HIPCHK(hipGetLastError());
 }
 
-void run_stage(exo_handle *h, int stage) {
-  launch_fluxes(h);
-  launch_update(h, stage);
-  launch_implicit(h, SS_BETA[stage - 1]);
-  launch_fill_ghosts(h);
+void run_stage(Launcher &L, int stage) {
+  launch_fluxes(L);
+  launch_update(L, stage);
+  launch_implicit(L, SS_BETA[stage - 1]);
+  launch_fill_ghosts(L);
+}
+
+void run_step(Launcher &L, double dt_fixed) {
+  launch_dt(L, dt_fixed);
+  for (int stg = 1; stg <= 3; ++stg) run_stage(L, stg);
+}
+
+// compensated conserved totals of every instance into out[ninst][5] (device)
+void launch_sums(Launcher &L, double *out) {
+  launch(L, K_CELLS, -1, k_sums, BLK, 0);
+  launch(L, K_ONE, -1, k_sums_final, BLK, 0, (const int *)(L.d_bstart + (size_t)K_CELLS * (L.ninst + 1)),
+         out);
+  HIPCHK(hipGetLastError());
+}
+
+// Autotuner of the persistent launch width: for each launch kind that only
+// reads the conserved state (so repeating it changes nothing), time each
+// candidate width and keep the fastest.  Widths are multiples of the CU count.
+void tune_launcher(Launcher &L, int reps) {
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  bool prof = L.prof;
+  L.prof = false;
+  const int kinds[] = {K_PRIM, K_GHOST_U, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX, K_CELLS};
+  for (int kind : kinds) {
+    if (L.nblk[kind] == 0) continue;
+    std::vector<int> cands = {0};
+    for (int m : {1, 2, 4, 8, 16})
+      if ((long long)m * ncu < L.nblk[kind]) cands.push_back(m * ncu);
+    float best = 1e30f;
+    int best_cap = 0;
+    for (int c : cands) {
+      L.cap[kind] = c;
+      for (int r = -1; r < reps; ++r) {  // r = -1 warms up
+        if (r == 0) HIPCHK(hipEventRecord(e0, L.stream));
+        switch (kind) {
+          case K_PRIM: launch(L, K_PRIM, -1, k_prim, 64, 0, DFLOOR, SFLOOR); break;
+          case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
+          case K_X1: launch(L, K_X1, -1, k_flux_x1, 64, L.lds_x1); break;
+          case K_X1_MIX: launch(L, K_X1_MIX, -1, k_flux_x1_mix, 64, L.lds_x1); break;
+          case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
+          case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
+          case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
+          case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1>, BLK, 0); break;
+          case K_CELLS: launch(L, K_CELLS, -1, k_dt, BLK, 0); break;
+        }
+      }
+      HIPCHK(hipEventRecord(e1, L.stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best * 0.97f) {  // prefer the plain launch unless clearly slower
+        best = ms;
+        best_cap = c;
+      }
+    }
+    L.cap[kind] = best_cap;
+  }
+  // kinds sharing a shape with a tuned one
+  L.cap[K_GHOST_S] = L.cap[K_GHOST_U];
+  L.cap[K_UPD] = L.cap[K_UPD_MIX] = L.cap[K_ADD_DU] = L.cap[K_CELLS];
+  L.prof = prof;
+  HIPCHK(hipEventDestroy(e0));
+  HIPCHK(hipEventDestroy(e1));
+  HIPCHK(hipGetLastError());
 }
 
 // copy between the host MeshBlock layout (p, n, k, j, i) and the device
@@ -1751,8 +2171,35 @@ void set_reference(exo_handle *h, const double *u) {
                         hipMemcpyHostToDevice, h->stream));
 }
 
-template <typename F>
-int guarded(exo_handle *h, F &&fn) {
+void prof_begin(Launcher &L) {
+  HIPCHK(hipStreamSynchronize(L.stream));
+  L.ev_used = 0;
+  L.ev_kind.clear();
+  L.prof = true;
+}
+
+void prof_end(Launcher &L, double *ms, int *counts, int nkinds) {
+  HIPCHK(hipStreamSynchronize(L.stream));
+  L.prof = false;
+  for (int q = 0; q < nkinds; ++q) {
+    if (ms) ms[q] = 0.;
+    if (counts) counts[q] = 0;
+  }
+  for (size_t e = 0; e < L.ev_kind.size(); ++e) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, L.ev_pool[2 * e], L.ev_pool[2 * e + 1]));
+    int kid = L.ev_kind[e];
+    if (kid < nkinds) {
+      if (ms) ms[kid] += t;
+      if (counts) counts[kid] += 1;
+    }
+  }
+  L.ev_used = 0;
+  L.ev_kind.clear();
+}
+
+template <typename T, typename F>
+int guarded(T *h, F &&fn) {
   if (!h) return EXO_ERR_ARG;
   try {
     fn();
@@ -1802,7 +2249,7 @@ int exo_create(const exo_config *cfg, exo_handle **out) {
 int exo_destroy(exo_handle *h) {
   if (!h) return EXO_ERR_ARG;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->L.ev_pool) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1826,7 +2273,7 @@ int exo_upload_state(exo_handle *h, const double *u, const double *s) {
     double dt0[2] = {0., DBL_MAX * 0.4};
     HIPCHK(hipMemcpyAsync(h->st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
     if (h->mixed) set_reference(h, u);
-    launch_fill_ghosts(h);
+    launch_fill_ghosts(h->L);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_state = true;
   });
@@ -1850,7 +2297,7 @@ int exo_download_prim(exo_handle *h, double *w) {
 int exo_calc_fluxes(exo_handle *h) {
   return guarded(h, [&] {
     if (!h->have_state) throw std::string("no state uploaded");
-    launch_fluxes(h);
+    launch_fluxes(h->L);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
@@ -1858,6 +2305,9 @@ int exo_calc_fluxes(exo_handle *h) {
 int exo_download_flux(exo_handle *h, int dir, double *out) {
   return guarded(h, [&] {
     if (dir < 0 || dir > 2 || !out) throw std::string("bad flux direction");
+    if (h->mixed)
+      throw std::string("the mixed form keeps FP32 perturbation fluxes only; "
+                        "download fluxes from an EXO_PREC_FP64 handle");
     Grid &g = h->g;
     std::vector<double> buf((size_t)g.nv * g.fvstride);
     HIPCHK(hipMemcpyAsync(buf.data(), h->st.flx[dir], buf.size() * sizeof(double),
@@ -1878,14 +2328,14 @@ int exo_add_flux_divergence(exo_handle *h, double wght) {
     // stage-1 weights with beta * dt = wght: u1 <- u, u <- u1, then the update
     double dtv = wght;
     HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    launch_update(h, 1);
+    launch_update(h->L, 1);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
 
 int exo_fill_ghosts(exo_handle *h) {
   return guarded(h, [&] {
-    launch_fill_ghosts(h);
+    launch_fill_ghosts(h->L);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
@@ -1894,14 +2344,14 @@ int exo_implicit_correct(exo_handle *h, double dt_stage) {
   return guarded(h, [&] {
     double dtv = dt_stage;
     HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    launch_implicit(h, 1.0);
+    launch_implicit(h->L, 1.0);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
 
 int exo_new_dt(exo_handle *h, double *dt) {
   return guarded(h, [&] {
-    launch_dt(h, 0.0);
+    launch_dt(h->L, 0.0);
     double v[2];
     HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1914,7 +2364,7 @@ int exo_stage(exo_handle *h, int stage, double dt) {
     if (stage < 1 || stage > 3) throw std::string("stage must be 1..3");
     double dtv = dt;
     HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    run_stage(h, stage);
+    run_stage(h->L, stage);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
@@ -1923,8 +2373,7 @@ int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
   return guarded(h, [&] {
     if (!h->have_state) throw std::string("no state uploaded");
     for (int s = 0; s < nsteps; ++s) {
-      launch_dt(h, dt_fixed);
-      for (int stg = 1; stg <= 3; ++stg) run_stage(h, stg);
+      run_step(h->L, dt_fixed);
     }
     if (dt_last) {
       double v[2];
@@ -1936,34 +2385,11 @@ int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
 }
 
 int exo_profile_begin(exo_handle *h) {
-  return guarded(h, [&] {
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->ev_used = 0;
-    h->ev_kind.clear();
-    h->prof = true;
-  });
+  return guarded(h, [&] { prof_begin(h->L); });
 }
 
 int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds) {
-  return guarded(h, [&] {
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->prof = false;
-    for (int q = 0; q < nkinds; ++q) {
-      if (ms) ms[q] = 0.;
-      if (counts) counts[q] = 0;
-    }
-    for (size_t e = 0; e < h->ev_kind.size(); ++e) {
-      float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, h->ev_pool[2 * e], h->ev_pool[2 * e + 1]));
-      int kid = h->ev_kind[e];
-      if (kid < nkinds) {
-        if (ms) ms[kid] += t;
-        if (counts) counts[kid] += 1;
-      }
-    }
-    h->ev_used = 0;
-    h->ev_kind.clear();
-  });
+  return guarded(h, [&] { prof_end(h->L, ms, counts, nkinds); });
 }
 
 int exo_sync(exo_handle *h) {
@@ -2078,6 +2504,160 @@ int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t
     std::memcpy(buf, src, n * esz);
   }
   return EXO_OK;
+}
+
+size_t exo_footprint(const exo_config *cfg) {
+  if (!cfg) return 0;
+  Setup s;
+  if (!build_setup(*cfg, &s).empty()) return 0;
+  return footprint_bytes(s);
+}
+
+int exo_conserved_sums(exo_handle *h, double *out5) {
+  return guarded(h, [&] {
+    if (!out5) throw std::string("null output");
+    if (!h->have_state) throw std::string("no state uploaded");
+    launch_sums(h->L, h->d_sums);
+    HIPCHK(hipMemcpyAsync(out5, h->d_sums, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_tune(exo_handle *h, int reps, int *caps, int ncaps) {
+  return guarded(h, [&] {
+    if (!h->have_state) throw std::string("no state uploaded");
+    tune_launcher(h->L, reps > 0 ? reps : 3);
+    for (int q = 0; caps && q < ncaps && q < K_NKIND; ++q) caps[q] = h->L.cap[q];
+  });
+}
+
+// --- batches ---------------------------------------------------------------
+namespace {
+void batch_free_graph(exo_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void batch_ready(exo_batch *b) {
+  for (exo_handle *m : b->members) {
+    if (!m->have_state) throw std::string("a batch member has no state uploaded");
+    HIPCHK(hipStreamSynchronize(m->stream));
+  }
+}
+}  // namespace
+
+int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **out) {
+  if (!members || n <= 0 || !out) return EXO_ERR_ARG;
+  *out = nullptr;
+  for (int i = 0; i < n; ++i)
+    if (!members[i]) return EXO_ERR_ARG;
+  exo_batch *b = new exo_batch();
+  b->members.assign(members, members + n);
+  b->use_graph = flags & EXO_BATCH_GRAPH ? 1 : 0;
+  std::vector<void *> allocs;
+  try {
+    HIPCHK(hipStreamCreateWithFlags(&b->L.stream, hipStreamNonBlocking));
+    b->L.per_lane = members[0]->L.per_lane;
+    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(members[i]->stream));
+    launcher_setup(b->L, b->members, allocs);
+    void *p;
+    HIPCHK(hipMalloc(&p, (size_t)n * 5 * sizeof(double)));
+    allocs.push_back(p);
+    b->d_sums = (double *)p;
+  } catch (const std::string &e) {
+    fprintf(stderr, "exo_batch_create: %s\n", e.c_str());
+    for (void *p : allocs) (void)hipFree(p);
+    if (b->L.stream) (void)hipStreamDestroy(b->L.stream);
+    delete b;
+    return EXO_ERR_HIP;
+  }
+  *out = b;
+  return EXO_OK;
+}
+
+int exo_batch_destroy(exo_batch *b) {
+  if (!b) return EXO_ERR_ARG;
+  if (b->L.stream) (void)hipStreamSynchronize(b->L.stream);
+  batch_free_graph(b);
+  for (hipEvent_t e : b->L.ev_pool) (void)hipEventDestroy(e);
+  (void)hipFree(b->L.d_inst);
+  (void)hipFree(b->L.d_bstart);
+  (void)hipFree(b->d_sums);
+  if (b->L.stream) (void)hipStreamDestroy(b->L.stream);
+  delete b;
+  return EXO_OK;
+}
+
+const char *exo_batch_last_error(const exo_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int exo_batch_step(exo_batch *b, int nsteps, double dt_fixed) {
+  return guarded(b, [&] {
+    if (nsteps < 0) throw std::string("nsteps must be >= 0");
+    batch_ready(b);
+    if (b->use_graph && !b->L.prof) {
+      if (!b->exec || b->graph_dt_fixed != dt_fixed) {
+        batch_free_graph(b);
+        HIPCHK(hipStreamBeginCapture(b->L.stream, hipStreamCaptureModeThreadLocal));
+        try {
+          run_step(b->L, dt_fixed);
+        } catch (...) {
+          hipGraph_t gr = nullptr;
+          (void)hipStreamEndCapture(b->L.stream, &gr);
+          if (gr) (void)hipGraphDestroy(gr);
+          throw;
+        }
+        HIPCHK(hipStreamEndCapture(b->L.stream, &b->graph));
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_dt_fixed = dt_fixed;
+      }
+      for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(b->exec, b->L.stream));
+    } else {
+      for (int s = 0; s < nsteps; ++s) run_step(b->L, dt_fixed);
+    }
+    HIPCHK(hipStreamSynchronize(b->L.stream));
+  });
+}
+
+int exo_batch_sums(exo_batch *b, double *out) {
+  return guarded(b, [&] {
+    if (!out) throw std::string("null output");
+    batch_ready(b);
+    launch_sums(b->L, b->d_sums);
+    HIPCHK(hipMemcpyAsync(out, b->d_sums, b->members.size() * 5 * sizeof(double),
+                          hipMemcpyDeviceToHost, b->L.stream));
+    HIPCHK(hipStreamSynchronize(b->L.stream));
+  });
+}
+
+int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps) {
+  return guarded(b, [&] {
+    batch_ready(b);
+    batch_free_graph(b);
+    tune_launcher(b->L, reps > 0 ? reps : 3);
+    for (int q = 0; caps && q < ncaps && q < K_NKIND; ++q) caps[q] = b->L.cap[q];
+  });
+}
+
+int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps) {
+  return guarded(b, [&] {
+    if (!caps) throw std::string("null caps");
+    batch_free_graph(b);
+    for (int q = 0; q < ncaps && q < K_NKIND; ++q) b->L.cap[q] = caps[q] > 0 ? caps[q] : 0;
+  });
+}
+
+int exo_batch_profile_begin(exo_batch *b) {
+  return guarded(b, [&] { prof_begin(b->L); });
+}
+
+int exo_batch_profile_end(exo_batch *b, double *ms, int *counts, int nkinds) {
+  return guarded(b, [&] { prof_end(b->L, ms, counts, nkinds); });
+}
+
+int exo_batch_sync(exo_batch *b) {
+  return guarded(b, [&] { HIPCHK(hipStreamSynchronize(b->L.stream)); });
 }
 
 }  // extern "C"

@@ -124,6 +124,88 @@ int exo_sync(exo_handle *h);
 int exo_profile_begin(exo_handle *h);
 int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
 
+/* --- conserved totals and launch tuning ---------------------------------- */
+/* Sum over the interior cells of u_n * CellVolume for every conserved
+ * variable (out5[n], n < NVAR; the rest 0) in FP64 with compensated
+ * (TwoSum) addition in a fixed order: deterministic, accurate to about one
+ * rounding of each total.  The reference has no such diagnostic; it checks the
+ * conservation of the finite-volume update (mass, and energy without sources). */
+int exo_conserved_sums(exo_handle *h, double *out5);
+/* Autotune the persistent launch width of the handle's kernels that only read
+ * the conserved state (repeated launches change nothing): each candidate
+ * width is timed `reps` times and the fastest kept.  caps (may be NULL)
+ * receives up to ncaps tuned widths, one per launch kind (0 = one workgroup
+ * per block). */
+int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
+
+/* Device bytes exo_create allocates for this configuration (host only; 0 for
+ * an invalid configuration).  Used to admit work by memory footprint. */
+size_t exo_footprint(const exo_config *cfg);
+
+/* --- batches of independent instances ----------------------------------- */
+/* A batch advances several handles (uneven sizes, any configuration) with one
+ * launch per phase over all of them: each kernel's blocks are split over the
+ * members by a prefix table and a launch may be persistent (fewer workgroups
+ * than blocks, looping).  With EXO_BATCH_GRAPH each SSPRK3 step is captured
+ * once as a HIP graph and replayed.  Every member keeps its own state, dt and
+ * CFL; results are identical to stepping each member alone.  Members must be
+ * idle (no call in flight) while a batch call runs, and must outlive the
+ * batch.  No reference counterpart: the reference runs one instance per
+ * process (main.cpp.in:130-186). */
+typedef struct exo_batch exo_batch;
+#define EXO_BATCH_GRAPH 1
+int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **out);
+int exo_batch_destroy(exo_batch *b);
+const char *exo_batch_last_error(const exo_batch *b);
+/* nsteps SSPRK3 steps of every member (dt from each member's CFL when
+ * dt_fixed <= 0); returns when the work has finished */
+int exo_batch_step(exo_batch *b, int nsteps, double dt_fixed);
+/* exo_conserved_sums of every member: out[member][5] */
+int exo_batch_sums(exo_batch *b, double *out);
+/* exo_tune over the batch's launches; exo_batch_set_caps sets widths directly */
+int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps);
+int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps);
+/* live kernel timing of the batch's launches (kernel ids as exo_profile_end;
+ * graphs are not used while profiling) */
+int exo_batch_profile_begin(exo_batch *b);
+int exo_batch_profile_end(exo_batch *b, double *ms, int *counts, int nkinds);
+int exo_batch_sync(exo_batch *b);
+
+/* --- serving: continuous batching of forward-run requests ---------------- */
+/* A request loop in the library: prep threads build initial states, one
+ * scheduler thread admits prepared requests in arrival order while their
+ * device footprint (exo_footprint) fits mem_budget bytes (0: 80 % of the free
+ * device memory) and fewer than max_active run, keeps their state resident in
+ * HBM and advances all active requests together through one exo_batch in
+ * chunks of at most chunk_steps steps (flags: EXO_BATCH_GRAPH).  Requests that
+ * arrive while others run join at the next chunk boundary; finished requests
+ * free their memory at once.  A request's result does not depend on the mix.
+ * No reference counterpart: the reference runs one instance per process. */
+typedef struct exo_server exo_server;
+int exo_server_create(size_t mem_budget, int max_active, int chunk_steps, int flags,
+                      int prep_threads, exo_server **out);
+/* Queue a forward run of nsteps SSPRK3 steps (CFL dt).  The initial state is
+ * copied from u0/s0 (host layout above), or, with u0 == NULL, built by
+ * exo_init_state_cfg(cfg, kind, seed, params).  keep_state = 0 drops the final
+ * state (only the info of exo_server_wait is kept).  Fails with EXO_ERR_NOMEM
+ * if the request can never fit the budget. */
+int exo_server_submit(exo_server *s, const exo_config *cfg, const double *u0, const double *s0,
+                      int kind, uint64_t seed, const double *params, int nparams, int nsteps,
+                      int keep_state, uint64_t *id_out);
+/* Wait (timeout_s <= 0: forever) for a request, copy its final state to
+ * u_out/s_out (may be NULL) and release it.  info[10] (may be NULL): latency,
+ * queue wait, run time (s), last dt, cells, steps, mass before/after, energy
+ * (shallow water: x2 momentum) total before/after. */
+int exo_server_wait(exo_server *s, uint64_t id, double timeout_s, double *u_out, double *s_out,
+                    double *info);
+/* stats[9]: completed, cell-updates, seconds first submit to last completion,
+ * seconds in batch steps, batches formed, chunks, peak active, peak bytes,
+ * budget bytes */
+int exo_server_stats(exo_server *s, double *stats, int n);
+const char *exo_server_last_error(const exo_server *s);
+/* Finishes the requests already on the device, drops queued ones. */
+int exo_server_destroy(exo_server *s);
+
 /* --- problem set-up (synthetic initial states of the benchmark configs) - */
 /* Fill host conserved arrays (interior cells) for a seeded initial state.
  * kind 0: shallow-water random-mode height field in geostrophic balance
@@ -152,7 +234,7 @@ int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out
 
 /* Host-only introspection of the set-up tables (no GPU needed): which 0 ghost
  * remap descriptors, 1 edge-exchange descriptors, 2 metric tables
- * [16][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f and 5 x3f of the six panels.
+ * [20][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f and 5 x3f of the six panels.
  * With buf == NULL only *count is returned. */
 int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap_bytes,
                  size_t *count);
