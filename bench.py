@@ -47,16 +47,35 @@ def workload_config(ex, precision):
 IC_PARAMS = (250.0, 1.0e5, 287.0, 0.01, 5.0)
 
 
-def algorithmic_bytes_per_cell(kernel, nv, s=8):
-    """Compulsory HBM bytes per interior cell and launch."""
+def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
+    """Compulsory HBM bytes per interior cell and launch: each array the
+    kernel must read or write once (state FP64; fluxes FP32 in the mixed form,
+    which also stores the FP64 x1 face pressure)."""
+    f = 4 if mixed else 8
     return {
-        "flux_x1": 2 * nv * s,      # read w column, write x1 face flux
-        "flux_x2": 2 * nv * s,      # read w, write x2 face flux
-        "flux_x3": 2 * nv * s,      # read w, write x3 face flux
-        "update": 9 * nv * s,       # read u, u1, w, F1, F2, F3; write u, u1, du
-        "implicit": 5 * nv * s,     # read du, w, u; write du, u
-        "prim": 2 * nv * s,         # read u, write w
+        "flux_x1": nv * 8 + nv * f + (8 if mixed else 0) + nt * (8 + f),
+        "flux_x2": nv * 8 + nv * f + nt * (8 + f),
+        "flux_x3": nv * 8 + nv * f + nt * (8 + f),
+        # read u, u1, w, F1, F2, F3 (+ psf); write u, u1, du; scalars s, s1, G1..3
+        "update": 6 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f),
+        "implicit": 5 * nv * 8,     # read du, w, u; write du, u
+        "prim": 2 * nv * 8 + 2 * nt * 8,  # read u (s), write w (r)
+        "dt": nv * 8,               # read w
     }.get(kernel)
+
+
+def kernel_rooflines(prof, cells, nv, mixed, nt=0):
+    """per kernel: average launch time and achieved algorithmic GB/s"""
+    out = {}
+    for k, (ms, n) in prof.items():
+        if n == 0:
+            continue
+        avg = ms / n
+        b = algorithmic_bytes_per_cell(k, nv, mixed, nt)
+        gbs = b * cells / (avg * 1e-3) / 1e9 if b else None
+        out[k] = {"ms_per_launch": avg, "launches": n, "bytes_per_cell": b, "GB_s": gbs,
+                  "hbm_frac": gbs / HBM_PEAK_GBS if gbs else None}
+    return out
 
 
 def rel_l2(a, b):
@@ -67,10 +86,12 @@ def rel_l2(a, b):
     return np.sqrt((d ** 2).sum(axis=(0, 2)) / (r ** 2).sum(axis=(0, 2)))
 
 
-def run_gpu(ex, precision, steps, warmup):
+def run_gpu(ex, precision, steps, warmup, perturb=0.0):
     cfg = workload_config(ex, precision)
     cells = 6 * cfg.nx * cfg.nx * cfg.nx1
     u0, s0 = ex.init_state(cfg, 1, SEED, IC_PARAMS)
+    if perturb:  # relative perturbation of the initial state (sensitivity floor)
+        u0 = u0 * (1.0 + perturb * np.random.default_rng(5).standard_normal(u0.shape))
     g = ex.Solver(cfg)
     g.upload(u0, s0)
     if warmup:
@@ -158,6 +179,36 @@ def ensemble(ex, precision, steps, n_inst=256, keep=16):
     }, states
 
 
+def large(ex, steps):
+    """6 x 384 x 384 x 128 rotating atmosphere, 2 tracers, HLLC, mixed form."""
+    cfg = ex.make_config(nx=384, nx1=128, eos=ex.EOS_IDEAL, rsolver=ex.RS_HLLC,
+                         implicit_flag=9, ntracer=2, coriolis=ex.COR_3D, cfl_x1=1,
+                         x1min=R_EARTH, x1max=R_EARTH + 30000.0, gamma=1.4, grav=-9.81,
+                         omega=7.292e-5, cfl=0.4, precision=ex.PREC_MIXED)
+    u0, s0 = ex.init_state(cfg, 2, SEED, (300.0, 6.5e-3, 200.0, 1.0e5, 287.0, 0.5, 3.0e-3, 0.5))
+    g = ex.Solver(cfg)
+    g.upload(u0, s0)
+    del u0, s0
+    g.step(1)
+    m0 = g.sums()[0]
+    g.sync()
+    g.profile_begin()
+    t0 = time.perf_counter()
+    dt = g.step(steps)
+    g.sync()
+    t1 = time.perf_counter()
+    prof = g.profile_end()
+    m1 = g.sums()[0]
+    g.close()
+    cells = 6 * 384 * 384 * 128
+    return {"cells": cells, "steps": steps, "value": cells * steps / (t1 - t0),
+            "unit": "cell-updates/s", "ms_per_step": (t1 - t0) * 1e3 / steps, "dt_last_s": dt,
+            "mass_rel_drift": abs(m1 - m0) / abs(m0),
+            "kernels": kernel_rooflines(prof, cells, 5, True, 2),
+            "workload": "6x384x384 x 128 levels, 5 vars + 2 tracers, lapse-rate atmosphere "
+                        "with rotation, WENO5+HLLC, implicit 5x5 FP64, mixed form"}
+
+
 def serving(ex, precision, nreq, keep=2):
     """Poisson request stream through the continuous-batching server."""
     rng = np.random.default_rng(SEED + 3)
@@ -208,6 +259,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--requests", type=int, default=16)
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -219,6 +271,12 @@ def main():
     rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup)
     # the same seeded run in both forms: accuracy of the mixed form vs FP64
     l2 = rel_l2(rmx["u"], r64["u"])
+    # what FP32 rounding of the input alone does to the FP64 result: the
+    # noisy atmosphere amplifies any perturbation, so this is the floor any
+    # FP32 computation sits on
+    rfl = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, perturb=2.0 ** -24)
+    floor = rel_l2(rfl["u"], r64["u"])
+    del rfl
     r = rmx
     ms_step = r["sec"] * 1e3 / args.steps
     value = r["cells"] * args.steps / r["sec"]
@@ -228,19 +286,16 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     tot_ms, cnt = prof[dom]
     avg_s = tot_ms / cnt * 1e-3
-    s_store = 4 if dom.startswith("flux") else 8
-    bpc = algorithmic_bytes_per_cell(dom, r["nv"])
-    if bpc and dom.startswith("flux"):
-        bpc = r["nv"] * 8 + r["nv"] * s_store  # FP64 state read, FP32 flux written
+    bpc = algorithmic_bytes_per_cell(dom, r["nv"], True)
     achieved = (bpc * r["cells"] / avg_s / 1e9) if bpc else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
             "traffic": None, "avg_launch_ms": tot_ms / cnt,
             "bytes_per_cell": bpc}
-    kernels = {k: {"ms_per_launch": v[0] / v[1], "launches": v[1],
-                   "share": v[0] / sum(x[0] for x in prof.values())} for k, v in prof.items()}
-    prof64 = {k: v for k, v in r64["prof"].items() if v[1] > 0}
-    kernels64 = {k: {"ms_per_launch": v[0] / v[1], "launches": v[1]} for k, v in prof64.items()}
+    kernels = kernel_rooflines(prof, r["cells"], r["nv"], True)
+    for k in kernels:
+        kernels[k]["share"] = prof[k][0] / sum(x[0] for x in prof.values())
+    kernels64 = kernel_rooflines(r64["prof"], r64["cells"], r64["nv"], False)
     cpu = None if args.no_cpu else cpu_baseline(workload_config, ex)
     ens = srv = None
     if not args.only_headline:
@@ -258,6 +313,7 @@ def main():
                "mixed_rel_l2_vs_fp64_max": l2s,
                "workload": "Poisson 40 req/s, 3D ideal gas + 2 tracers, N 32..128, 32..96 "
                            "levels, WENO5+LMARS, implicit 5x5, 200 steps each"}
+    big = large(ex, args.large_steps) if args.large_steps > 0 and not args.only_headline else None
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
@@ -279,11 +335,15 @@ def main():
         "mixed_over_fp64": value / value64,
         "mixed_rel_l2_vs_fp64": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"), l2)},
         "mixed_rel_l2_max": float(l2.max()),
+        "fp64_input_rounding_floor": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"),
+                                                                 floor)},
+        "mixed_over_floor_max": float((l2 / floor).max()),
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
         "ensemble": ens,
         "serving": srv,
+        "large": big,
     }
     print(json.dumps(out))
 

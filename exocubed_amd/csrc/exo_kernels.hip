@@ -723,26 +723,30 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
   M5 Aprev;
   V5 dprev;
   size_t cs30 = (size_t)g.c(0, p, k, 0, j);  // column base in the coefficient array
+  // divisions by dt and by 2 vol share one correctly rounded reciprocal each
+  double ydt = 1. / dt;
+  M5 Jm = flux_jacobian(gm1c, wm), J0 = flux_jacobian(gm1c, w0);
   for (int i = is; i <= ie; ++i) {
     wproj(i + 1, wp);
     roe_average(prim, gm1, w0, wp);
     cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
     M5 Ap = abs_a(prim, cs, gm1);
-    M5 Jm = flux_jacobian(gm1c, wm), J0 = flux_jacobian(gm1c, w0), Jp = flux_jacobian(gm1c, wp);
+    M5 Jp = flux_jacobian(gm1c, wp);
     double aleft = g.area1(p, k, j, i), aright = g.area1(p, k, j, i + 1);
     double v = g.volume(p, k, j, i);
+    double v2 = 2. * v, yv2 = 1. / v2;
     M5 a, b, c;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        double dtq = r == q ? 1. / dt : 0.;
+        double dtq = r == q ? ydt : 0.;
         double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
-        a.a[r][q] = (Am.a[r][q] * aleft + Ap.a[r][q] * aright + (aright - aleft) * J0.a[r][q]) /
-                        (2. * v) +
+        a.a[r][q] = div_rn(Am.a[r][q] * aleft + Ap.a[r][q] * aright + (aright - aleft) * J0.a[r][q],
+                           v2, yv2) +
                     dtq - phi;
-        b.a[r][q] = -(Am.a[r][q] + Jm.a[r][q]) * aleft / (2. * v);
-        c.a[r][q] = -(Ap.a[r][q] - Jp.a[r][q]) * aright / (2. * v);
+        b.a[r][q] = div_rn(-(Am.a[r][q] + Jm.a[r][q]) * aleft, v2, yv2);
+        c.a[r][q] = div_rn(-(Ap.a[r][q] - Jp.a[r][q]) * aright, v2, yv2);
       }
     if (i == is)
 #pragma unroll
@@ -756,7 +760,7 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
         for (int q = 0; q < 5; ++q) a.a[r][q] += c.a[r][q] * (q == IVX ? -1. : 1.);
     V5 rhs;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) rhs.a[n] = st.du[g.c(n, p, k, i, j)] / dt;
+    for (int n = 0; n < 5; ++n) rhs.a[n] = div_rn(st.du[g.c(n, p, k, i, j)], dt, ydt);
     M5 ainv;
     V5 delta;
     if (i == is) {
@@ -786,6 +790,8 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
 #pragma unroll
     for (int n = 0; n < 5; ++n) cf[(size_t)(25 + n) * g.vstride + cs30 + (size_t)i * g.jp] = delta.a[n];
     Am = Ap;
+    Jm = J0;
+    J0 = Jp;
     for (int n = 0; n < 5; ++n) {
       wm[n] = w0[n];
       w0[n] = wp[n];
@@ -1646,9 +1652,15 @@ __global__ void k_weno_batch(const double *in5, double *out, int n) {
   out[t] = weno5(a[0], a[1], a[2], a[3], a[4]);
 }
 
+// op 0 exp, 1 log, 2 pow(x, y), 3 div_rn(x[2t], x[2t+1], 1 / x[2t+1])
 __global__ void k_math_batch(int op, const double *x, double y, double *out, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  if (op == 3) {
+    double a = x[2 * t], d = x[2 * t + 1];
+    out[t] = div_rn(a, d, 1.0 / d);
+    return;
+  }
   out[t] = op == 0 ? dexp(x[t]) : (op == 1 ? dlog(x[t]) : dpow(x[t], y));
 }
 
@@ -1807,6 +1819,16 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     tab[(size_t)kind * (n + 1) + n] = (int)acc;
     L.nblk[kind] = (int)acc;
   }
+  {
+    int dev = 0, maxlds = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    bool need_x1 = L.nblk[K_X1] > 0 || L.nblk[K_X1_MIX] > 0;
+    if (need_x1 && L.lds_x1 > (size_t)maxlds)
+      throw std::string("nx1 too large: the x1 column kernel needs ") +
+          std::to_string(L.lds_x1) + " bytes of LDS, the device allows " +
+          std::to_string(maxlds);
+  }
   void *p;
   HIPCHK(hipMalloc(&p, sizeof(Inst) * n));
   allocs.push_back(p);
@@ -1953,8 +1975,9 @@ void init_handle(exo_handle *h) {
   }
   if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
   {
-    const char *e = getenv("EXO_IMPLICIT_PER_LANE");
-    h->L.per_lane = e && e[0] == '1';
+    // one column per lane unless EXO_IMPLICIT_GROUP=1 (5 lanes per column)
+    const char *e = getenv("EXO_IMPLICIT_GROUP");
+    h->L.per_lane = !(e && e[0] == '1');
   }
   if (h->mixed) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
@@ -2443,12 +2466,14 @@ int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n) {
 int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n) {
   return guarded(h, [&] {
     if (n <= 0) return;
-    if (op < 0 || op > 2) throw std::string("bad op");
+    if (op < 0 || op > 3) throw std::string("bad op");
+    size_t nin = (size_t)n * (op == 3 ? 2 : 1);
     double *d;
-    HIPCHK(hipMalloc(&d, (size_t)2 * n * sizeof(double)));
-    HIPCHK(hipMemcpyAsync(d, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_math_batch, dim3(grid1(n)), dim3(BLK), 0, h->stream, op, d, y, d + n, n);
-    HIPCHK(hipMemcpyAsync(out, d + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+    HIPCHK(hipMalloc(&d, (nin + n) * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(d, x, nin * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_math_batch, dim3(grid1(n)), dim3(BLK), 0, h->stream, op, d, y, d + nin,
+                       n);
+    HIPCHK(hipMemcpyAsync(out, d + nin, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                           h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(d));

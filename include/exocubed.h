@@ -229,7 +229,8 @@ int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *
                      const double *wr, double *flx);
 /* WENO5 value at the face between b and c seen from cell c (device). */
 int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
-/* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y) */
+/* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y), or
+ * (op 3) the shared-reciprocal division x[2t] / x[2t+1] the implicit solve uses */
 int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n);
 
 /* Host-only introspection of the set-up tables (no GPU needed): which 0 ghost

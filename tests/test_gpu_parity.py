@@ -52,6 +52,25 @@ def test_math_device_matches_oracle(ex, orc):
         assert np.array_equal(out, ref), op
 
 
+def test_device_shared_reciprocal_division_is_ieee(ex):
+    """the implicit solve's x / d from RN(1/d) plus one FMA correction equals
+    the IEEE quotient on the device (random operands, hard significands)"""
+    rng = np.random.default_rng(7)
+    n = 400000
+    x = rng.standard_normal(n) * np.exp2(rng.integers(-60, 61, n))
+    d = rng.standard_normal(n) * np.exp2(rng.integers(-60, 61, n))
+    hard = np.array([1.0, np.nextafter(2.0, 0.0), np.nextafter(1.0, 2.0), 1.5,
+                     np.nextafter(1.0, 0.0), 3.0, 1.0 + 2.0 ** -52 * 3])
+    d[::5] = rng.choice(hard, len(d[::5])) * np.exp2(rng.integers(-30, 31, len(d[::5])))
+    x[1::5] = rng.choice(hard, len(x[1::5])) * np.exp2(rng.integers(-30, 31, len(x[1::5])))
+    pairs = np.ascontiguousarray(np.stack([x, d], axis=1).ravel())
+    g = ex.Solver(sw_config(ex))
+    out = np.zeros(n)
+    assert g.L.exo_math_batch(g.h, 3, pairs, 0.0, out, n) == 0
+    g.close()
+    assert np.array_equal(out, x / d)
+
+
 @pytest.mark.parametrize("rsolver", [0, 1])
 def test_shallow_water_fluxes(ex, orc, rsolver):
     cfg = sw_config(ex, nx=8, rsolver=rsolver)
