@@ -693,6 +693,21 @@ __global__ void __launch_bounds__(BLK) k_update(Work w, StageW sw) {
 // forward_backward.hpp:15-126), one thread per column; the forward sweep keeps
 // the previous block row in registers and spills a[i] and delta[i] to the
 // coefficient array for the backward substitution.  Ends with u += du.
+// FluxJacobian with e = (gm1 + 1) / gm1 * pres / rho given (one division per
+// row, carried between levels instead of three 5x5 matrices)
+__device__ __forceinline__ M5 jac_e(double gm1, const double *w, double e) {
+  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ];
+  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
+  double c1 = ((gm1 - 1) * s2 / 2 - e) * v1;
+  double c2 = e + s2 / 2 - gm1 * v1 * v1;
+  M5 J = {{{0, 1., 0., 0., 0.},
+           {gm1 * s2 / 2 - v1 * v1, (2. - gm1) * v1, -gm1 * v2, -gm1 * v3, gm1},
+           {-v1 * v2, v2, v1, 0., 0.},
+           {-v1 * v3, v3, 0., v1, 0.},
+           {c1, c2, -gm1 * v2 * v1, -gm1 * v3 * v1, (gm1 + 1) * v1}}};
+  return J;
+}
+
 __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, int lb, double beta) {
   int N = g.N;
   int nbx = (N * N + 63) / 64;
@@ -703,124 +718,176 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
   int is = g.is, ie = g.ie;
   double dt = beta * st.dtv[0];
   double gamma = g.gamma, grav = g.grav;
+  // column constants: Grid::area1 = ((DX2A DX3A) x1f) x1f SINC and
+  // Grid::volume = ((((dx1f DX2A) DX3A) x1v) x1v) SINC with the same roundings
   double cth = g.T(T_COSC, p, k, j), sth = g.T(T_SINC, p, k, j);
-  auto wproj = [&](int i, double *w) {
-    for (int n = 0; n < 5; ++n) w[n] = st.w[g.c(n, p, k, i, j)];
+  double dx2a = g.T(T_DX2A, p, k, j), dx3a = g.T(T_DX3A, p, k, j);
+  double a23 = dx2a * dx3a;
+  const double *__restrict__ x1f = g.x1f, *__restrict__ x1v = g.x1v, *__restrict__ dx1f = g.dx1f;
+  const double *__restrict__ W = st.w;
+  double *__restrict__ DU = st.du;
+  double *__restrict__ U = st.u;
+  double *__restrict__ CF = st.coef;
+  size_t vs = g.vstride, jp = g.jp;
+  size_t c0 = g.c(0, p, k, 0, j);  // column base (variable 0, i = 0)
+  auto wload = [&](int i, double *w) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) w[n] = W[n * vs + c0 + (size_t)i * jp];
+  };
+  auto wproj = [&](double *w) {
     w[IVY] += w[IVZ] * cth;
     w[IVZ] *= sth;
   };
   double gm1c = (gamma - 1.) * 1. / 1.;
   double gm1 = 0.5 * (gm1c + gm1c);
+  double k1 = (gm1c + 1) / gm1c;
+  auto erow = [&](const double *w) { return k1 * w[IPR] / w[IDN]; };
   // |A| at face is-1/2: the reference's row is-1 (full_correction.cpp:272-325)
   // leaves Am = R|L|R^-1 of the Roe state of (w(is-1), w(is))
   double wm[5], w0[5], wp[5], prim[5];
-  wproj(is - 1, wm);
-  wproj(is, w0);
+  wload(is - 1, wm);
+  wproj(wm);
+  wload(is, w0);
+  wproj(w0);
   roe_average(prim, gm1, wm, w0);
   double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
   M5 Am = abs_a(prim, cs, gm1);
-  // wm = w(i-1), w0 = w(i), wp = w(i+1) at row i
-  M5 Aprev;
-  V5 dprev;
-  size_t cs30 = (size_t)g.c(0, p, k, 0, j);  // column base in the coefficient array
+  double em = erow(wm), e0 = erow(w0);
   // divisions by dt and by 2 vol share one correctly rounded reciprocal each
   double ydt = 1. / dt;
-  M5 Jm = flux_jacobian(gm1c, wm), J0 = flux_jacobian(gm1c, w0);
+  M5 Aprev;
+  V5 dprev;
+  // level i+1 state and level i tendency, loaded one level ahead
+  double wnx[5], dun[5];
+  wload(is + 1, wnx);
+#pragma unroll
+  for (int n = 0; n < 5; ++n) dun[n] = DU[n * vs + c0 + (size_t)is * jp];
   for (int i = is; i <= ie; ++i) {
-    wproj(i + 1, wp);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) wp[n] = wnx[n];
+    double dui[5];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) dui[n] = dun[n];
+    if (i < ie) {  // prefetch the next level
+      wload(i + 2, wnx);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) dun[n] = DU[n * vs + c0 + (size_t)(i + 1) * jp];
+    }
+    wproj(wp);
+    double ep = erow(wp);
+    double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
+    double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
+    double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
+    double v2 = 2. * v, yv2 = 1. / v2;
+    // b, then b Aprev and b dprev, so b dies early
+    M5 b;
+    {
+      M5 Jm = jac_e(gm1c, wm, em);
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+          b.a[r][q] = div_rn(-(Am.a[r][q] + Jm.a[r][q]) * aleft, v2, yv2);
+    }
+    M5 ba;
+    V5 bd;
+    if (i > is) {
+      ba = m5_mul(b, Aprev);
+      bd = m5_mulv(b, dprev);
+    }
     roe_average(prim, gm1, w0, wp);
     cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
     M5 Ap = abs_a(prim, cs, gm1);
-    M5 Jp = flux_jacobian(gm1c, wp);
-    double aleft = g.area1(p, k, j, i), aright = g.area1(p, k, j, i + 1);
-    double v = g.volume(p, k, j, i);
-    double v2 = 2. * v, yv2 = 1. / v2;
-    M5 a, b, c;
-#pragma unroll
-    for (int r = 0; r < 5; ++r)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        double dtq = r == q ? ydt : 0.;
-        double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
-        a.a[r][q] = div_rn(Am.a[r][q] * aleft + Ap.a[r][q] * aright + (aright - aleft) * J0.a[r][q],
-                           v2, yv2) +
-                    dtq - phi;
-        b.a[r][q] = div_rn(-(Am.a[r][q] + Jm.a[r][q]) * aleft, v2, yv2);
-        c.a[r][q] = div_rn(-(Ap.a[r][q] - Jp.a[r][q]) * aright, v2, yv2);
-      }
-    if (i == is)
+    M5 c;
+    {
+      M5 Jp = jac_e(gm1c, wp, ep);
 #pragma unroll
       for (int r = 0; r < 5; ++r)
 #pragma unroll
-        for (int q = 0; q < 5; ++q) a.a[r][q] += b.a[r][q] * (q == IVX ? -1. : 1.);
-    if (i == ie)
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) a.a[r][q] += c.a[r][q] * (q == IVX ? -1. : 1.);
-    V5 rhs;
-#pragma unroll
-    for (int n = 0; n < 5; ++n) rhs.a[n] = div_rn(st.du[g.c(n, p, k, i, j)], dt, ydt);
-    M5 ainv;
-    V5 delta;
-    if (i == is) {
-      ainv = m5_inv(a);
-      delta = m5_mulv(ainv, rhs);
-    } else {
-      M5 ba = m5_mul(b, Aprev);
-      M5 tt;
-#pragma unroll
-      for (int r = 0; r < 5; ++r)
-#pragma unroll
-        for (int q = 0; q < 5; ++q) tt.a[r][q] = a.a[r][q] - ba.a[r][q];
-      ainv = m5_inv(tt);
-      V5 bd = m5_mulv(b, dprev);
-      V5 tv;
-#pragma unroll
-      for (int n = 0; n < 5; ++n) tv.a[n] = rhs.a[n] - bd.a[n];
-      delta = m5_mulv(ainv, tv);
+        for (int q = 0; q < 5; ++q)
+          c.a[r][q] = div_rn(-(Ap.a[r][q] - Jp.a[r][q]) * aright, v2, yv2);
     }
+    M5 T;
+    {
+      M5 J0 = jac_e(gm1c, w0, e0);
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          double dtq = r == q ? ydt : 0.;
+          double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
+          double a = div_rn(Am.a[r][q] * aleft + Ap.a[r][q] * aright +
+                                (aright - aleft) * J0.a[r][q],
+                            v2, yv2) +
+                     dtq - phi;
+          if (i == is) a += b.a[r][q] * (q == IVX ? -1. : 1.);  // reflecting bottom
+          if (i == ie) a += c.a[r][q] * (q == IVX ? -1. : 1.);  // reflecting top
+          T.a[r][q] = i > is ? a - ba.a[r][q] : a;
+        }
+    }
+    V5 tv;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      double rhs = div_rn(dui[n], dt, ydt);
+      tv.a[n] = i > is ? rhs - bd.a[n] : rhs;
+    }
+    M5 ainv = m5_inv(T);
+    V5 delta = m5_mulv(ainv, tv);
     Aprev = m5_mul(ainv, c);
     dprev = delta;
-    double *cf = st.coef;
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp] = Aprev.a[r][q];
+      for (int q = 0; q < 5; ++q) CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)i * jp] = Aprev.a[r][q];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) cf[(size_t)(25 + n) * g.vstride + cs30 + (size_t)i * g.jp] = delta.a[n];
+    for (int n = 0; n < 5; ++n) CF[(size_t)(25 + n) * vs + c0 + (size_t)i * jp] = delta.a[n];
     Am = Ap;
-    Jm = J0;
-    J0 = Jp;
+    em = e0;
+    e0 = ep;
+#pragma unroll
     for (int n = 0; n < 5; ++n) {
       wm[n] = w0[n];
       w0[n] = wp[n];
     }
   }
-  // backward substitution; delta(ie) is final
+  // backward substitution; delta(ie) is final.  The coefficients of level
+  // i - 1 are loaded while level i is finished.
   V5 dnext = dprev;
+#pragma unroll
   for (int n = 0; n < 5; ++n) {
-    size_t cc = g.c(n, p, k, ie, j);
-    st.du[cc] = dnext.a[n];
-    st.u[cc] += dnext.a[n];
+    size_t cc = n * vs + c0 + (size_t)ie * jp;
+    DU[cc] = dnext.a[n];
+    U[cc] += dnext.a[n];
   }
-  double *cf = st.coef;
-  for (int i = ie - 1; i >= is; --i) {
-    M5 A;
-    V5 d;
+  M5 An;
+  V5 dn;
+  if (ie - 1 >= is) {
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) A.a[r][q] = cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp];
+      for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(ie - 1) * jp];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) d.a[n] = cf[(size_t)(25 + n) * g.vstride + cs30 + (size_t)i * g.jp];
+    for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(ie - 1) * jp];
+  }
+  for (int i = ie - 1; i >= is; --i) {
+    M5 A = An;
+    V5 d = dn;
+    if (i - 1 >= is) {
+#pragma unroll
+      for (int r = 0; r < 5; ++r)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(i - 1) * jp];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(i - 1) * jp];
+    }
     V5 ad = m5_mulv(A, dnext);
 #pragma unroll
     for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
+#pragma unroll
     for (int n = 0; n < 5; ++n) {
-      size_t cc = g.c(n, p, k, i, j);
-      st.du[cc] = d.a[n];
-      st.u[cc] += d.a[n];
+      size_t cc = n * vs + c0 + (size_t)i * jp;
+      DU[cc] = d.a[n];
+      U[cc] += d.a[n];
     }
     dnext = d;
   }
