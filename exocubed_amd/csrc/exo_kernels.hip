@@ -33,8 +33,25 @@ using namespace exo;
 namespace {
 
 // ---------------------------------------------------------------------------
-// kernel-side view of one instance
-struct Grid {
+// Kernel-side view of one instance.  AS is the address space of the array
+// pointers: 0 (generic) where the host fills the structures, 1 (global) in the
+// copies the kernels work on, so that every array access is a global (not a
+// flat) memory instruction.
+#define ADDR(a) __attribute__((address_space(a)))
+
+// pointer to T in address space AS (0: the language's default pointer)
+template <int AS, typename T>
+struct APtr {
+  using type = ADDR(AS) T *;
+};
+template <typename T>
+struct APtr<0, T> {
+  using type = T *;
+};
+#define PTR(T) typename APtr<AS, T>::type
+
+template <int AS>
+struct GridT {
   int N, nx1, nc1, nc2, nc3, jp, nv, nt;
   int is, ie, js, je, ks, ke;
   int eos, rs, coriolis, implicit, cfl_x1;
@@ -42,8 +59,8 @@ struct Grid {
   size_t plane, pstride, vstride;      // cell arrays
   size_t fplane, fpstride, fvstride;   // face arrays
   // geometry
-  const double *x1f, *x1v, *dx1f;
-  const double *tab;                   // [T_NTAB][6][nc3][nc2]
+  PTR(const double) x1f, x1v, dx1f;
+  PTR(const double) tab;               // [T_NTAB][6][nc3][nc2]
   size_t t2stride;                     // 6*nc3*nc2
   __host__ __device__ size_t c(int v, int p, int k, int i, int j) const {
     return v * vstride + p * pstride + k * plane + (size_t)i * jp + j;
@@ -69,22 +86,64 @@ struct Grid {
   }
 };
 
-struct State {
-  double *u, *w, *u1, *du;       // nv cell arrays
-  double *s, *r, *s1;            // nt cell arrays
-  double *flx[3], *sflx[3];      // face arrays
-  double *edge;                  // [nv][6][4][N][nx1] ghost-side face states
-  double *coef;                  // implicit column coefficients, 30 per cell
-  unsigned long long *dtbits;    // CFL minimum as ordered bits
-  double *dtv;                   // [0] dt of the step, [1] previous dt
+template <int AS>
+struct StateT {
+  PTR(double) u, w, u1, du;             // nv cell arrays
+  PTR(double) s, r, s1;                 // nt cell arrays
+  PTR(double) flx[3], sflx[3];          // face arrays
+  PTR(double) edge;                     // [nv][6][4][N][nx1] ghost-side face states
+  PTR(double) coef;                     // implicit column coefficients, 30 per cell
+  PTR(unsigned long long) dtbits;       // CFL minimum as ordered bits
+  PTR(double) dtv;                      // [0] dt of the step, [1] previous dt
   // mixed precision: FP32 perturbation fluxes, FP64 reference parts
-  float *fflx[3], *fsflx[3];     // face fluxes without the reference pressure
-  float *fedge;                  // ghost-side edge states (perturbations)
-  double *psf;                   // FP64 hydrostatic x1 face pressure [p][k][i][j]
-  double *lref;                  // per level: [0,nc1) split pressure P, [nc1,2nc1) the
-                                 // reconstructed variable's reference, [2nc1,3nc1) rho_ref
-  double *part;                  // per-block compensated partial sums [block][5][2]
+  PTR(float) fflx[3], fsflx[3];         // face fluxes without the reference pressure
+  PTR(float) fedge;                     // ghost-side edge states (perturbations)
+  PTR(double) psf;                      // FP64 hydrostatic x1 face pressure [p][k][i][j]
+  PTR(double) lref;                     // per level: [0,nc1) split pressure P, [nc1,2nc1)
+                                        // the reconstructed variable's reference,
+                                        // [2nc1,3nc1) rho_ref
+  PTR(double) part;                     // per-block compensated partial sums [block][5][2]
 };
+
+using Grid = GridT<0>;
+using State = StateT<0>;
+using DGrid = GridT<1>;
+using DState = StateT<1>;
+
+template <typename T>
+__device__ __forceinline__ ADDR(1) T *glb(T *p) {
+  return (ADDR(1) T *)p;
+}
+
+__device__ __forceinline__ DGrid to_global(const Grid &h) {
+  DGrid g;
+  g.N = h.N; g.nx1 = h.nx1; g.nc1 = h.nc1; g.nc2 = h.nc2; g.nc3 = h.nc3; g.jp = h.jp;
+  g.nv = h.nv; g.nt = h.nt;
+  g.is = h.is; g.ie = h.ie; g.js = h.js; g.je = h.je; g.ks = h.ks; g.ke = h.ke;
+  g.eos = h.eos; g.rs = h.rs; g.coriolis = h.coriolis; g.implicit = h.implicit;
+  g.cfl_x1 = h.cfl_x1;
+  g.gamma = h.gamma; g.grav = h.grav; g.cfl = h.cfl;
+  g.plane = h.plane; g.pstride = h.pstride; g.vstride = h.vstride;
+  g.fplane = h.fplane; g.fpstride = h.fpstride; g.fvstride = h.fvstride;
+  g.x1f = glb(h.x1f); g.x1v = glb(h.x1v); g.dx1f = glb(h.dx1f); g.tab = glb(h.tab);
+  g.t2stride = h.t2stride;
+  return g;
+}
+
+__device__ __forceinline__ DState to_global(const State &h) {
+  DState s;
+  s.u = glb(h.u); s.w = glb(h.w); s.u1 = glb(h.u1); s.du = glb(h.du);
+  s.s = glb(h.s); s.r = glb(h.r); s.s1 = glb(h.s1);
+  for (int d = 0; d < 3; ++d) {
+    s.flx[d] = glb(h.flx[d]);
+    s.sflx[d] = glb(h.sflx[d]);
+    s.fflx[d] = glb(h.fflx[d]);
+    s.fsflx[d] = glb(h.fsflx[d]);
+  }
+  s.edge = glb(h.edge); s.coef = glb(h.coef); s.dtbits = glb(h.dtbits); s.dtv = glb(h.dtv);
+  s.fedge = glb(h.fedge); s.psf = glb(h.psf); s.lref = glb(h.lref); s.part = glb(h.part);
+  return s;
+}
 
 constexpr int BLK = 256;
 
@@ -101,8 +160,10 @@ struct Inst {
   int ngd, ned;
 };
 
+// The instance list itself is a separate __restrict__ kernel argument: no
+// store of a kernel can then alias it, so the pointers read from it are known
+// to be global memory and are accessed with global (not flat) instructions.
 struct Work {
-  const Inst *inst;    // [ninst]
   const int *bstart;   // [ninst + 1] first block of each instance
   int ninst, nblk;
 };
@@ -118,83 +179,102 @@ __device__ __forceinline__ int find_inst(const Work &w, int b) {
   return lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup h runs on
+// XCD h mod 8), each with its own L2.  xcd_order gives XCD x the contiguous
+// range [x q + min(x, r), ...) of the n blocks (q = n / 8, r = n mod 8), so
+// neighbouring rows and planes of a stencil are read through one L2.  It is a
+// bijection of [0, n).
+__device__ __forceinline__ int xcd_order(int h, int n) {
+  int q = n >> 3, r = n & 7, x = h & 7;
+  return x * q + (x < r ? x : r) + (h >> 3);
+}
+
 // for each block of the work list this workgroup owns: g, st and the block's
 // index lb within its instance
 #define WORK_LOOP(w)                                                     \
-  for (int b_ = blockIdx.x; b_ < (w).nblk; b_ += gridDim.x)
+  for (int h_ = blockIdx.x; h_ < (w).nblk; h_ += gridDim.x)
 #define WORK_INST(w)                                                     \
+  int b_ = xcd_order(h_, (w).nblk);                                      \
   int ii_ = find_inst((w), b_);                                          \
-  const Grid g = (w).inst[ii_].g;                                        \
-  const State st = (w).inst[ii_].st;                                     \
+  const DGrid g = to_global(I_[ii_].g);                                  \
+  const DState st = to_global(I_[ii_].st);                               \
   int lb = b_ - (w).bstart[ii_]
 
 // ---------------------------------------------------------------------------
 // EquationOfState::ConservedToPrimitive (shallow_yz_hydro.cpp:42-67,
-// ideal_moist_hydro.cpp:77-159) on interior and face-ghost columns, then the
-// reflecting x1 boundary (v1 mirrored).  One thread per (panel, k, j) column.
-__device__ __forceinline__ void prim_body(const Grid &g, const State &st, int lb, double dfloor,
+// ideal_moist_hydro.cpp:77-159) on the interior levels of interior and
+// face-ghost columns, then the reflecting x1 boundary (v1 mirrored): one
+// thread per cell; a cell within NG levels of either end also writes the
+// ghost it mirrors into (ghost is - q mirrors is + q - 1, ghost ie + q mirrors
+// ie - q + 1).  With nx1 < NG the ghosts beyond the mirror of an interior cell
+// are not written (only shallow water allows it, and it never reads x1 ghosts).
+__device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int lb, double dfloor,
                                           double sfloor) {
-  int nbx = (g.nc2 + 63) / 64;
-  int j = (lb % nbx) * 64 + threadIdx.x;
-  int k = (lb / nbx) % g.nc3, p = lb / (nbx * g.nc3);
-  if (j >= g.nc2) return;
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int ni = g.nx1;
+  size_t total = (size_t)6 * g.nc3 * ni * g.nc2;
+  if (t >= total) return;
+  int j = (int)(t % g.nc2);
+  size_t rr0 = t / g.nc2;
+  int i = g.is + (int)(rr0 % ni);
+  rr0 /= ni;
+  int k = (int)(rr0 % g.nc3), p = (int)(rr0 / g.nc3);
   bool kin = k >= g.ks && k <= g.ke, jin = j >= g.js && j <= g.je;
   if (!kin && !jin) return;  // corner ghosts are never read
   double cth = g.T(T_COSC, p, k, j);
-  for (int i = g.is; i <= g.ie; ++i) {
-    if (g.eos == EOS_SW) {
-      double ud = st.u[g.c(IDN, p, k, i, j)];
-      ud = (ud > dfloor) ? ud : dfloor;
-      st.u[g.c(IDN, p, k, i, j)] = ud;
-      double v = st.u[g.c(IVY, p, k, i, j)], ww = st.u[g.c(IVZ, p, k, i, j)];
-      double di = 1. / ud;
-      double sth2 = 1. - cth * cth;
-      double wy = v / sth2 - ww * cth / sth2;
-      double wz = -v * cth / sth2 + ww / sth2;
-      st.w[g.c(IDN, p, k, i, j)] = ud;
-      st.w[g.c(IVX, p, k, i, j)] = 0.;
-      st.w[g.c(IVY, p, k, i, j)] = wy * di;
-      st.w[g.c(IVZ, p, k, i, j)] = wz * di;
-    } else {
-      double gm1 = g.gamma - 1.0;
-      double u_d = st.u[g.c(IDN, p, k, i, j)], u_m1 = st.u[g.c(IVX, p, k, i, j)];
-      double u_m2 = st.u[g.c(IVY, p, k, i, j)], u_m3 = st.u[g.c(IVZ, p, k, i, j)];
-      double u_e = st.u[g.c(IEN, p, k, i, j)];
-      double density = 0.;
-      density += u_d;
-      double di = 1. / density;
-      double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
-      double sth2 = 1. - cth * cth;
-      double v = w_vy, ww = w_vz;
-      w_vy = v / sth2 - ww * cth / sth2;
-      w_vz = -v * cth / sth2 + ww / sth2;
-      double KE = 0.5 * (u_m1 * w_vx + u_m2 * w_vy + u_m3 * w_vz);
-      st.w[g.c(IDN, p, k, i, j)] = density;
-      st.w[g.c(IVX, p, k, i, j)] = w_vx;
-      st.w[g.c(IVY, p, k, i, j)] = w_vy;
-      st.w[g.c(IVZ, p, k, i, j)] = w_vz;
-      st.w[g.c(IPR, p, k, i, j)] = gm1 * (u_e - KE) * 1. / 1.;
-    }
-    for (int n = 0; n < g.nt; ++n) {
-      double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / st.u[g.c(IDN, p, k, i, j)]);
-      st.r[g.c(n, p, k, i, j)] = rr < sfloor ? sfloor : rr;
-    }
+  double wv[5] = {0., 0., 0., 0., 0.};
+  if (g.eos == EOS_SW) {
+    double ud = st.u[g.c(IDN, p, k, i, j)];
+    ud = (ud > dfloor) ? ud : dfloor;
+    st.u[g.c(IDN, p, k, i, j)] = ud;
+    double v = st.u[g.c(IVY, p, k, i, j)], ww = st.u[g.c(IVZ, p, k, i, j)];
+    double di = 1. / ud;
+    double sth2 = 1. - cth * cth;
+    double wy = v / sth2 - ww * cth / sth2;
+    double wz = -v * cth / sth2 + ww / sth2;
+    wv[IDN] = ud;
+    wv[IVX] = 0.;
+    wv[IVY] = wy * di;
+    wv[IVZ] = wz * di;
+  } else {
+    double gm1 = g.gamma - 1.0;
+    double u_d = st.u[g.c(IDN, p, k, i, j)], u_m1 = st.u[g.c(IVX, p, k, i, j)];
+    double u_m2 = st.u[g.c(IVY, p, k, i, j)], u_m3 = st.u[g.c(IVZ, p, k, i, j)];
+    double u_e = st.u[g.c(IEN, p, k, i, j)];
+    double density = 0.;
+    density += u_d;
+    double di = 1. / density;
+    double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
+    double sth2 = 1. - cth * cth;
+    double v = w_vy, ww = w_vz;
+    w_vy = v / sth2 - ww * cth / sth2;
+    w_vz = -v * cth / sth2 + ww / sth2;
+    double KE = 0.5 * (u_m1 * w_vx + u_m2 * w_vy + u_m3 * w_vz);
+    wv[IDN] = density;
+    wv[IVX] = w_vx;
+    wv[IVY] = w_vy;
+    wv[IVZ] = w_vz;
+    wv[IPR] = gm1 * (u_e - KE) * 1. / 1.;
   }
-  for (int n = 0; n < g.nv; ++n)
-    for (int q = 1; q <= NG; ++q) {
-      double lo = st.w[g.c(n, p, k, g.is + q - 1, j)];
-      double hi = st.w[g.c(n, p, k, g.ie - q + 1, j)];
-      st.w[g.c(n, p, k, g.is - q, j)] = n == IVX ? -lo : lo;
-      st.w[g.c(n, p, k, g.ie + q, j)] = n == IVX ? -hi : hi;
-    }
-  for (int n = 0; n < g.nt; ++n)
-    for (int q = 1; q <= NG; ++q) {
-      st.r[g.c(n, p, k, g.is - q, j)] = st.r[g.c(n, p, k, g.is + q - 1, j)];
-      st.r[g.c(n, p, k, g.ie + q, j)] = st.r[g.c(n, p, k, g.ie - q + 1, j)];
-    }
+  int qb = i - g.is + 1, qt = g.ie - i + 1;  // mirrors into ghosts is - qb, ie + qt
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
+    st.w[g.c(n, p, k, i, j)] = wv[n];
+    double m = n == IVX ? -wv[n] : wv[n];
+    if (qb <= NG) st.w[g.c(n, p, k, g.is - qb, j)] = m;
+    if (qt <= NG) st.w[g.c(n, p, k, g.ie + qt, j)] = m;
+  }
+  for (int n = 0; n < g.nt; ++n) {
+    double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / st.u[g.c(IDN, p, k, i, j)]);
+    rr = rr < sfloor ? sfloor : rr;
+    st.r[g.c(n, p, k, i, j)] = rr;
+    if (qb <= NG) st.r[g.c(n, p, k, g.is - qb, j)] = rr;
+    if (qt <= NG) st.r[g.c(n, p, k, g.ie + qt, j)] = rr;
+  }
 }
 
-__global__ void __launch_bounds__(64) k_prim(Work w, double dfloor, double sfloor) {
+__global__ void __launch_bounds__(BLK) k_prim(const Inst *__restrict__ I_, Work w, double dfloor, double sfloor) {
   WORK_LOOP(w) {
     WORK_INST(w);
     prim_body(g, st, lb, dfloor, sfloor);
@@ -205,13 +285,14 @@ __global__ void __launch_bounds__(64) k_prim(Work w, double dfloor, double sfloo
 // Panel-edge ghost remap (CubedSphereUtility::PackData with InteprolateX2/X3,
 // cubed_sphere_utility.cpp:92-617): one thread per (ghost cell, i); the
 // descriptor carries the interpolation weights and vector-map points.
-__device__ __forceinline__ void ghost_body(const Grid &g, int lb, const GhostDesc *__restrict__ gd,
-                                           int ndesc, double *arr, int nvar, int vectors) {
+__device__ __forceinline__ void ghost_body(const DGrid &g, int lb,
+                                           const ADDR(1) GhostDesc *__restrict__ gd, int ndesc,
+                                           ADDR(1) double *arr, int nvar, int vectors) {
   int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= ndesc * ni) return;
   int di = t / ni, i = g.is + t % ni;
-  GhostDesc d = gd[di];
+  GhostDesc d = *(const GhostDesc *)(gd + di);
   int sp = d.src_panel;
   size_t o0, o1;
   if (d.along_x2) {
@@ -251,12 +332,12 @@ __device__ __forceinline__ void ghost_body(const Grid &g, int lb, const GhostDes
 }
 
 // which = 0: conserved variables (vectors remapped), 1: passive scalars
-__global__ void __launch_bounds__(BLK) k_ghost(Work w, int which) {
+__global__ void __launch_bounds__(BLK) k_ghost(const Inst *__restrict__ I_, Work w, int which) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    const Inst &I = w.inst[ii_];
-    if (which == 0) ghost_body(g, lb, I.gd, I.ngd, st.u, g.nv, 1);
-    else ghost_body(g, lb, I.gd, I.ngd, st.s, g.nt, 0);
+    const Inst &I = I_[ii_];
+    if (which == 0) ghost_body(g, lb, glb(I.gd), I.ngd, st.u, g.nv, 1);
+    else ghost_body(g, lb, glb(I.gd), I.ngd, st.s, g.nt, 0);
   }
 }
 
@@ -266,7 +347,7 @@ __global__ void __launch_bounds__(BLK) k_ghost(Work w, int which) {
 // (change_to_perturbation.cpp:39-190) around Weno5X1, then the x1 Riemann
 // solve (patches/19.calculate_fluxes.patch:77-106).  The face pressure psf
 // lives in LDS, [face][lane].
-__device__ __forceinline__ void flux_x1_body(const Grid &g, const State &st, int lb, double *lds) {
+__device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, int lb, double *lds) {
   int lane = threadIdx.x;
   int N = g.N;
   int nbx = (N * N + 63) / 64;
@@ -358,7 +439,7 @@ __device__ __forceinline__ void flux_x1_body(const Grid &g, const State &st, int
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64) k_flux_x1(Work w) {
+__global__ void __launch_bounds__(64) k_flux_x1(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
@@ -371,13 +452,13 @@ __global__ void __launch_bounds__(64) k_flux_x1(Work w) {
 // 326-576): the interior-side WENO state of each edge face, its IVY / IVZ
 // mapped through lat-lon into the neighbour's frame, stored in the
 // neighbour's ghost-side slot [n][panel][side][a][i].
-__device__ __forceinline__ void edge_body(const Grid &g, const State &st, int lb,
-                                          const EdgeDesc *__restrict__ ed, int nedge) {
+__device__ __forceinline__ void edge_body(const DGrid &g, const DState &st, int lb,
+                                          const ADDR(1) EdgeDesc *__restrict__ ed, int nedge) {
   int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= nedge * ni) return;
   int e = t / ni, i = g.is + t % ni;
-  EdgeDesc d = ed[e];
+  EdgeDesc d = *(const EdgeDesc *)(ed + e);
   int p = d.src_panel, k = d.src_k, j = d.src_j;
   double q[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -414,10 +495,10 @@ __device__ __forceinline__ void edge_body(const Grid &g, const State &st, int lb
     if (n < g.nv) st.edge[n * vs + base] = q[n];
 }
 
-__global__ void __launch_bounds__(BLK) k_edge_states(Work w) {
+__global__ void __launch_bounds__(BLK) k_edge_states(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    edge_body(g, st, lb, w.inst[ii_].ed, w.inst[ii_].ned);
+    edge_body(g, st, lb, glb(I_[ii_].ed), I_[ii_].ned);
   }
 }
 
@@ -425,7 +506,7 @@ __global__ void __launch_bounds__(BLK) k_edge_states(Work w) {
 // edge's ghost side from the exchanged states, and Hydro::RiemannSolver with
 // the cubed-sphere frame maps.  One thread per (panel, k, i, j) face.
 template <int dir>
-__device__ __forceinline__ void flux_h_body(const Grid &g, const State &st, int lb) {
+__device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, int lb) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   int na = N + 1;  // faces along the normal axis
@@ -499,7 +580,7 @@ __device__ __forceinline__ void flux_h_body(const Grid &g, const State &st, int 
     cth = g.T(T_CTH3, p, k, j);
   }
   riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
-  double *F = st.flx[dir];
+  ADDR(1) double *F = st.flx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
     if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
@@ -515,7 +596,7 @@ __device__ __forceinline__ void flux_h_body(const Grid &g, const State &st, int 
 }
 
 template <int dir>
-__global__ void __launch_bounds__(BLK) k_flux_h(Work w) {
+__global__ void __launch_bounds__(BLK) k_flux_h(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
     flux_h_body<dir>(g, st, lb);
@@ -535,7 +616,7 @@ struct StageW {
 };
 
 template <bool MIX>
-__device__ __forceinline__ void update_body(const Grid &g, const State &st, int lb, StageW sw) {
+__device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
@@ -553,14 +634,28 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
   double a2l = g.area2(p, k, j, i), a2r = g.area2(p, k, j + 1, i);
   double a3l = g.area3(p, k, j, i), a3r = g.area3(p, k + 1, j, i);
   double vol = g.volume(p, k, j, i);
-  for (int n = 0; n < g.nv; ++n) {
+  // the first stage only sets u1 = delta u (u itself is unchanged when delta
+  // is 1), later stages keep u1 unless delta != 0: arrays that would be read
+  // only to be overwritten, or rewritten unchanged, are not touched
+  bool keep_u = sw.first && sw.delta == 1.0;
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (n >= g.nv) continue;
     size_t c = g.c(n, p, k, i, j);
-    double u = st.u[c], u1 = st.u1[c];
-    if (sw.first) u1 = 0.;
-    if (sw.delta != 0.0) u1 += sw.delta * u;
-    if (sw.first) u = u1;
-    else u = sw.g1 * u + sw.g2 * u1;
-    st.u1[c] = u1;
+    double u = st.u[c], u1;
+    if (sw.first) {
+      u1 = 0.;
+      if (sw.delta != 0.0) u1 += sw.delta * u;
+      u = u1;
+      st.u1[c] = u1;
+    } else {
+      u1 = st.u1[c];
+      if (sw.delta != 0.0) {
+        u1 += sw.delta * u;
+        st.u1[c] = u1;
+      }
+      u = sw.g1 * u + sw.g2 * u1;
+    }
     double dflx;
     if (!MIX) {
       if (g.eos == EOS_SW) dflx = 0.;
@@ -570,7 +665,7 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
       dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
     } else {
       // FP32 perturbation fluxes plus the FP64 reference-pressure parts
-      const float *F0 = st.fflx[0], *F1 = st.fflx[1], *F2 = st.fflx[2];
+      const ADDR(1) float *F0 = st.fflx[0], *F1 = st.fflx[1], *F2 = st.fflx[2];
       if (g.eos == EOS_SW) dflx = 0.;
       else
         dflx = a1r * (double)F0[g.f(n, p, k, i + 1, j)] - a1l * (double)F0[g.f(n, p, k, i, j)];
@@ -587,7 +682,7 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
                  (a3r * g.T(T_PRZ3, p, k + 1, j) - a3l * g.T(T_PRZ3, p, k, j))) * P;
     }
     if (tend) {
-      st.u[c] = u;
+      if (!keep_u) st.u[c] = u;
       out[n] = 0.;
       out[n] -= wght * dflx / vol;
     } else {
@@ -650,17 +745,25 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
       out[IVZ] += wght * a3c;
     }
   }
-  double *dst = tend ? st.du : st.u;
+  ADDR(1) double *dst = tend ? st.du : st.u;
   for (int n = 0; n < g.nv; ++n) dst[g.c(n, p, k, i, j)] = out[n];
   // passive scalars: weighted average and flux divergence into s
   for (int n = 0; n < g.nt; ++n) {
     size_t c = g.c(n, p, k, i, j);
-    double s = st.s[c], s1 = st.s1[c];
-    if (sw.first) s1 = 0.;
-    if (sw.delta != 0.0) s1 += sw.delta * s;
-    if (sw.first) s = s1;
-    else s = sw.g1 * s + sw.g2 * s1;
-    st.s1[c] = s1;
+    double s = st.s[c], s1;
+    if (sw.first) {
+      s1 = 0.;
+      if (sw.delta != 0.0) s1 += sw.delta * s;
+      s = s1;
+      st.s1[c] = s1;
+    } else {
+      s1 = st.s1[c];
+      if (sw.delta != 0.0) {
+        s1 += sw.delta * s;
+        st.s1[c] = s1;
+      }
+      s = sw.g1 * s + sw.g2 * s1;
+    }
     double dflx;
     if (!MIX) {
       dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
@@ -668,7 +771,7 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
       dflx += a2r * st.sflx[1][g.f(n, p, k, i, j + 1)] - a2l * st.sflx[1][g.f(n, p, k, i, j)];
       dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
     } else {
-      const float *S0 = st.fsflx[0], *S1 = st.fsflx[1], *S2 = st.fsflx[2];
+      const ADDR(1) float *S0 = st.fsflx[0], *S1 = st.fsflx[1], *S2 = st.fsflx[2];
       dflx = g.eos == EOS_SW ? 0.
                              : a1r * (double)S0[g.f(n, p, k, i + 1, j)] -
                                    a1l * (double)S0[g.f(n, p, k, i, j)];
@@ -681,7 +784,7 @@ __device__ __forceinline__ void update_body(const Grid &g, const State &st, int 
 }
 
 template <bool MIX>
-__global__ void __launch_bounds__(BLK) k_update(Work w, StageW sw) {
+__global__ void __launch_bounds__(BLK) k_update(const Inst *__restrict__ I_, Work w, StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
     update_body<MIX>(g, st, lb, sw);
@@ -708,7 +811,21 @@ __device__ __forceinline__ M5 jac_e(double gm1, const double *w, double e) {
   return J;
 }
 
-__device__ __forceinline__ void implicit_body(const Grid &g, const State &st, int lb, double beta) {
+constexpr int IMPL_MAXNC1 = 512;  // radial cells incl. ghosts the column solve supports
+
+// geo: LDS for the instance's x1f | x1v | dx1f, so the per-level geometry is
+// read without waiting on the vector-memory counter (which also counts the
+// coefficient stores and the prefetches in flight)
+__device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, int lb, double beta,
+                                              double *geo) {
+  for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
+    geo[q] = g.x1f[q];
+    if (q < g.nc1) {
+      geo[IMPL_MAXNC1 + 1 + q] = g.x1v[q];
+      geo[2 * (IMPL_MAXNC1 + 1) + q] = g.dx1f[q];
+    }
+  }
+  __syncthreads();
   int N = g.N;
   int nbx = (N * N + 63) / 64;
   int col = (lb % nbx) * 64 + threadIdx.x;
@@ -723,11 +840,11 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
   double cth = g.T(T_COSC, p, k, j), sth = g.T(T_SINC, p, k, j);
   double dx2a = g.T(T_DX2A, p, k, j), dx3a = g.T(T_DX3A, p, k, j);
   double a23 = dx2a * dx3a;
-  const double *__restrict__ x1f = g.x1f, *__restrict__ x1v = g.x1v, *__restrict__ dx1f = g.dx1f;
-  const double *__restrict__ W = st.w;
-  double *__restrict__ DU = st.du;
-  double *__restrict__ U = st.u;
-  double *__restrict__ CF = st.coef;
+  const double *x1f = geo, *x1v = geo + IMPL_MAXNC1 + 1, *dx1f = geo + 2 * (IMPL_MAXNC1 + 1);
+  const ADDR(1) double *__restrict__ W = st.w;
+  ADDR(1) double *__restrict__ DU = st.du;
+  ADDR(1) double *__restrict__ U = st.u;
+  ADDR(1) double *__restrict__ CF = st.coef;
   size_t vs = g.vstride, jp = g.jp;
   size_t c0 = g.c(0, p, k, 0, j);  // column base (variable 0, i = 0)
   auto wload = [&](int i, double *w) {
@@ -757,24 +874,28 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
   double ydt = 1. / dt;
   M5 Aprev;
   V5 dprev;
-  // level i+1 state and level i tendency, loaded one level ahead
-  double wnx[5], dun[5];
-  wload(is + 1, wnx);
+  // The next level's state and tendency are loaded one level ahead and put
+  // to use (projected, divided by dt) before the level's coefficient stores
+  // are issued, so a level never waits for the previous level's stores.
+  double wnext[5], rnext[5], enext;
+  wload(is + 1, wnext);
+  wproj(wnext);
+  enext = erow(wnext);
 #pragma unroll
-  for (int n = 0; n < 5; ++n) dun[n] = DU[n * vs + c0 + (size_t)is * jp];
+  for (int n = 0; n < 5; ++n) rnext[n] = div_rn(DU[n * vs + c0 + (size_t)is * jp], dt, ydt);
   for (int i = is; i <= ie; ++i) {
+    double rhsv[5], wnx[5], dun[5];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) wp[n] = wnx[n];
-    double dui[5];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) dui[n] = dun[n];
+    for (int n = 0; n < 5; ++n) {
+      wp[n] = wnext[n];
+      rhsv[n] = rnext[n];
+    }
+    double ep = enext;
     if (i < ie) {  // prefetch the next level
       wload(i + 2, wnx);
 #pragma unroll
       for (int n = 0; n < 5; ++n) dun[n] = DU[n * vs + c0 + (size_t)(i + 1) * jp];
     }
-    wproj(wp);
-    double ep = erow(wp);
     double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
     double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
     double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
@@ -827,14 +948,19 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
     }
     V5 tv;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      double rhs = div_rn(dui[n], dt, ydt);
-      tv.a[n] = i > is ? rhs - bd.a[n] : rhs;
-    }
+    for (int n = 0; n < 5; ++n) tv.a[n] = i > is ? rhsv[n] - bd.a[n] : rhsv[n];
     M5 ainv = m5_inv(T);
     V5 delta = m5_mulv(ainv, tv);
     Aprev = m5_mul(ainv, c);
     dprev = delta;
+    if (i < ie) {  // use the prefetched next level before this level's stores
+#pragma unroll
+      for (int n = 0; n < 5; ++n) wnext[n] = wnx[n];
+      wproj(wnext);
+      enext = erow(wnext);
+#pragma unroll
+      for (int n = 0; n < 5; ++n) rnext[n] = div_rn(dun[n], dt, ydt);
+    }
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
@@ -860,7 +986,7 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
     U[cc] += dnext.a[n];
   }
   M5 An;
-  V5 dn;
+  V5 dn, un;
   if (ie - 1 >= is) {
 #pragma unroll
     for (int r = 0; r < 5; ++r)
@@ -868,17 +994,21 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
       for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(ie - 1) * jp];
 #pragma unroll
     for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(ie - 1) * jp];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) un.a[n] = U[n * vs + c0 + (size_t)(ie - 1) * jp];
   }
   for (int i = ie - 1; i >= is; --i) {
     M5 A = An;
-    V5 d = dn;
-    if (i - 1 >= is) {
+    V5 d = dn, uo = un;
+    if (i - 1 >= is) {  // the next level's coefficients and state, one level ahead
 #pragma unroll
       for (int r = 0; r < 5; ++r)
 #pragma unroll
         for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(i - 1) * jp];
 #pragma unroll
       for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(i - 1) * jp];
+#pragma unroll
+      for (int n = 0; n < 5; ++n) un.a[n] = U[n * vs + c0 + (size_t)(i - 1) * jp];
     }
     V5 ad = m5_mulv(A, dnext);
 #pragma unroll
@@ -887,16 +1017,18 @@ __device__ __forceinline__ void implicit_body(const Grid &g, const State &st, in
     for (int n = 0; n < 5; ++n) {
       size_t cc = n * vs + c0 + (size_t)i * jp;
       DU[cc] = d.a[n];
-      U[cc] += d.a[n];
+      U[cc] = uo.a[n] + d.a[n];
     }
     dnext = d;
   }
 }
 
-__global__ void __launch_bounds__(64) k_implicit(Work w, double beta) {
+__global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Work w, double beta) {
+  __shared__ double geo[3 * (IMPL_MAXNC1 + 1)];
   WORK_LOOP(w) {
     WORK_INST(w);
-    implicit_body(g, st, lb, beta);
+    implicit_body(g, st, lb, beta, geo);
+    __syncthreads();  // geo is refilled by the next block of the loop
   }
 }
 
@@ -930,24 +1062,28 @@ __device__ __forceinline__ void absa_row(int r, const double *prim, double cs, d
   if (write_ri) {
     // R^-1 row r (rows 3, 4 are constants but written all the same)
     double ri[5];
+    double d1 = 2. * cs * hp, d2 = 2. * hp;
     if (r == 0) {
-      ri[0] = (cs * ke + u * hp) / (2. * cs * hp);
-      ri[1] = (-hp - cs * u) / (2. * cs * hp);
-      ri[2] = -v / (2. * hp);
-      ri[3] = -w / (2. * hp);
-      ri[4] = 1. / (2. * hp);
+      double y1 = 1. / d1, y2 = 1. / d2;
+      ri[0] = div_rn(cs * ke + u * hp, d1, y1);
+      ri[1] = div_rn(-hp - cs * u, d1, y1);
+      ri[2] = div_rn(-v, d2, y2);
+      ri[3] = div_rn(-w, d2, y2);
+      ri[4] = y2;
     } else if (r == 1) {
-      ri[0] = (hp - ke) / hp;
-      ri[1] = u / hp;
-      ri[2] = v / hp;
-      ri[3] = w / hp;
-      ri[4] = -1. / hp;
+      double y3 = 1. / hp;
+      ri[0] = div_rn(hp - ke, hp, y3);
+      ri[1] = div_rn(u, hp, y3);
+      ri[2] = div_rn(v, hp, y3);
+      ri[3] = div_rn(w, hp, y3);
+      ri[4] = div_rn(-1., hp, y3);
     } else if (r == 2) {
-      ri[0] = (cs * ke - u * hp) / (2. * cs * hp);
-      ri[1] = (hp - cs * u) / (2. * cs * hp);
-      ri[2] = -v / (2. * hp);
-      ri[3] = -w / (2. * hp);
-      ri[4] = 1. / (2. * hp);
+      double y1 = 1. / d1, y2 = 1. / d2;
+      ri[0] = div_rn(cs * ke - u * hp, d1, y1);
+      ri[1] = div_rn(hp - cs * u, d1, y1);
+      ri[2] = div_rn(-v, d2, y2);
+      ri[3] = div_rn(-w, d2, y2);
+      ri[4] = y2;
     } else if (r == 3) {
       ri[0] = -v; ri[1] = 0.; ri[2] = 1.; ri[3] = 0.; ri[4] = 0.;
     } else {
@@ -987,7 +1123,7 @@ __device__ __forceinline__ void jac_row(int r, double gm1, const double *w, doub
 // LU with partial pivoting of the 5x5 in T (repeated per lane) and column
 // `col` of the inverse, m5_inv's arithmetic
 __device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) {
-  double lu[5][5];
+  double lu[5][5], rd[5];
   int piv[5];
 #pragma unroll
   for (int r = 0; r < 5; ++r) {
@@ -1019,9 +1155,11 @@ __device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) 
         piv[r] = t;
       }
     }
+    double yc = 1.0 / lu[c][c];
+    rd[c] = yc;
 #pragma unroll
     for (int r = c + 1; r < 5; ++r) {
-      double l = lu[r][c] / lu[c][c];
+      double l = div_rn(lu[r][c], lu[c][c], yc);
       lu[r][c] = l;
 #pragma unroll
       for (int q = c + 1; q < 5; ++q) lu[r][q] = lu[r][q] - l * lu[c][q];
@@ -1040,11 +1178,11 @@ __device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) 
     double t = y[r];
 #pragma unroll
     for (int q = r + 1; q < 5; ++q) t = t - lu[r][q] * xcol[q];
-    xcol[r] = t / lu[r][r];
+    xcol[r] = div_rn(t, lu[r][r], rd[r]);
   }
 }
 
-__device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st, int lb,
+__device__ __forceinline__ void implicit_grp_body(const DGrid &g, const DState &st, int lb,
                                                   double beta, double *lds, double *lap) {
   int lane = threadIdx.x;
   int grp = lane / IG, r = lane % IG;
@@ -1070,6 +1208,7 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
   };
   double gm1c = (gamma - 1.) * 1. / 1.;
   double gm1 = 0.5 * (gm1c + gm1c);
+  double ydt = 1. / dt;
   double wm[5], w0[5], wp[5], prim[5];
   wproj(is - 1, wm);
   wproj(is, w0);
@@ -1094,14 +1233,16 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
     jac_row(r, gm1c, wp, Jp);
     double aleft = act ? g.area1(p, k, j, i) : 1., aright = act ? g.area1(p, k, j, i + 1) : 1.;
     double v = act ? g.volume(p, k, j, i) : 1.;
+    double v2 = 2. * v, yv2 = 1. / v2;
     double a[5], b[5], c[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      double dtq = r == q ? 1. / dt : 0.;
+      double dtq = r == q ? ydt : 0.;
       double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
-      a[q] = (Am[q] * aleft + Ap[q] * aright + (aright - aleft) * J0[q]) / (2. * v) + dtq - phi;
-      b[q] = -(Am[q] + Jm[q]) * aleft / (2. * v);
-      c[q] = -(Ap[q] - Jp[q]) * aright / (2. * v);
+      a[q] = div_rn(Am[q] * aleft + Ap[q] * aright + (aright - aleft) * J0[q], v2, yv2) + dtq -
+             phi;
+      b[q] = div_rn(-(Am[q] + Jm[q]) * aleft, v2, yv2);
+      c[q] = div_rn(-(Ap[q] - Jp[q]) * aright, v2, yv2);
     }
     if (i == is)
 #pragma unroll
@@ -1109,7 +1250,7 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
     if (i == ie)
 #pragma unroll
       for (int q = 0; q < 5; ++q) a[q] += c[q] * (q == IVX ? -1. : 1.);
-    double rhs = act ? st.du[g.c(r, p, k, i, j)] / dt : 0.;
+    double rhs = act ? div_rn(st.du[g.c(r, p, k, i, j)], dt, ydt) : 0.;
     double T[5], tv;
     if (i == is) {
 #pragma unroll
@@ -1164,7 +1305,7 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
       Ldp[r] = d;
     }
     if (act) {
-      double *cf = st.coef;
+      ADDR(1) double *cf = st.coef;
 #pragma unroll
       for (int q = 0; q < 5; ++q)
         cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp] = An[q];
@@ -1185,7 +1326,7 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
     st.du[cc] = dn;
     st.u[cc] += dn;
   }
-  double *cf = st.coef;
+  ADDR(1) double *cf = st.coef;
   for (int i = ie - 1; i >= is; --i) {
     double A[5], d = 0.;
     if (act) {
@@ -1211,7 +1352,7 @@ __device__ __forceinline__ void implicit_grp_body(const Grid &g, const State &st
   }
 }
 
-__global__ void __launch_bounds__(64) k_implicit_grp(Work w, double beta) {
+__global__ void __launch_bounds__(64) k_implicit_grp(const Inst *__restrict__ I_, Work w, double beta) {
   __shared__ double lds[IGPW * ILDS];
   __shared__ double lap[IGPW * 25];
   WORK_LOOP(w) {
@@ -1222,7 +1363,7 @@ __global__ void __launch_bounds__(64) k_implicit_grp(Work w, double beta) {
 }
 
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
-__device__ __forceinline__ void add_du_body(const Grid &g, const State &st, int lb) {
+__device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, int lb) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
@@ -1238,7 +1379,7 @@ __device__ __forceinline__ void add_du_body(const Grid &g, const State &st, int 
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_add_du(Work w) {
+__global__ void __launch_bounds__(BLK) k_add_du(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
     add_du_body(g, st, lb);
@@ -1250,7 +1391,7 @@ __global__ void __launch_bounds__(BLK) k_add_du(Work w) {
 // width / signal speed, block minimum, device-wide minimum on the IEEE bits
 // (positive doubles order like unsigned integers).
 // one block per instance (work list of one block each), thread 0 acts
-__global__ void k_dt_reset(Work w) {
+__global__ void k_dt_reset(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
     (void)g;
@@ -1259,19 +1400,24 @@ __global__ void k_dt_reset(Work w) {
   }
 }
 
-__device__ __forceinline__ void dt_body(const Grid &g, const State &st, int lb, double *wmin) {
-  size_t t = (size_t)lb * BLK + threadIdx.x;
+constexpr int DTC = 8;  // cells per thread of the CFL minimum (fewer block atomics)
+
+__device__ __forceinline__ void dt_body(const DGrid &g, const DState &st, int lb, double *wmin) {
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
   double m = HUGE_VAL;
-  if (t < total) {
+  for (int cc = 0; cc < DTC; ++cc) {
+    size_t t = ((size_t)lb * DTC + cc) * BLK + threadIdx.x;
+    if (t >= total) break;
     int j = g.js + (int)(t % N);
     size_t r = t / N;
     int i = g.is + (int)(r % ni);
     r /= ni;
     int k = g.ks + (int)(r % N), p = (int)(r / N);
-    double wi[5];
-    for (int n = 0; n < g.nv; ++n) wi[n] = st.w[g.c(n, p, k, i, j)];
+    double wi[5] = {0., 0., 0., 0., 0.};
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (n < g.nv) wi[n] = st.w[g.c(n, p, k, i, j)];
     double cs = g.eos == EOS_SW ? sqrt(wi[IDN])
                                 : sqrt((1. + (g.gamma - 1) * 1. / 1.) * wi[IPR] / wi[IDN]);
     double dt1 = g.dx1f[i];
@@ -1296,11 +1442,11 @@ __device__ __forceinline__ void dt_body(const Grid &g, const State &st, int lb, 
   if (threadIdx.x == 0) {
     double b = wmin[0];
     for (int q = 1; q < (int)(blockDim.x >> 6); ++q) b = wmin[q] < b ? wmin[q] : b;
-    atomicMin(st.dtbits, (unsigned long long)__double_as_longlong(b));
+    atomicMin((unsigned long long *)st.dtbits, (unsigned long long)__double_as_longlong(b));
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_dt(Work w) {
+__global__ void __launch_bounds__(BLK) k_dt(const Inst *__restrict__ I_, Work w) {
   __shared__ double wmin[BLK / 64];
   WORK_LOOP(w) {
     WORK_INST(w);
@@ -1310,7 +1456,7 @@ __global__ void __launch_bounds__(BLK) k_dt(Work w) {
 }
 
 // dt = min(cfl * min_cells, 2 dt_prev) (Mesh::NewTimeStep growth limit)
-__global__ void k_dt_finish(Work w, double dt_fixed) {
+__global__ void k_dt_finish(const Inst *__restrict__ I_, Work w, double dt_fixed) {
   WORK_LOOP(w) {
     WORK_INST(w);
     (void)lb;
@@ -1374,7 +1520,7 @@ __device__ __forceinline__ void block_sum5(Pair *acc, double *lds) {
   __syncthreads();
 }
 
-__device__ __forceinline__ void sums_body(const Grid &g, const State &st, int lb, double *lds) {
+__device__ __forceinline__ void sums_body(const DGrid &g, const DState &st, int lb, double *lds) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
@@ -1402,7 +1548,7 @@ __device__ __forceinline__ void sums_body(const Grid &g, const State &st, int lb
 }
 
 // cell-shaped work list (one block per BLK interior cells)
-__global__ void __launch_bounds__(BLK) k_sums(Work w) {
+__global__ void __launch_bounds__(BLK) k_sums(const Inst *__restrict__ I_, Work w) {
   __shared__ double lds[(BLK / 64) * 5 * 2];
   WORK_LOOP(w) {
     WORK_INST(w);
@@ -1412,7 +1558,7 @@ __global__ void __launch_bounds__(BLK) k_sums(Work w) {
 
 // one block per instance; cstart is the cell-shaped table (partials per
 // instance), out[instance][5] receives the rounded compensated totals
-__global__ void __launch_bounds__(BLK) k_sums_final(Work w, const int *__restrict__ cstart,
+__global__ void __launch_bounds__(BLK) k_sums_final(const Inst *__restrict__ I_, Work w, const int *__restrict__ cstart,
                                                    double *out) {
   __shared__ double lds[(BLK / 64) * 5 * 2];
   WORK_LOOP(w) {
@@ -1444,7 +1590,49 @@ __global__ void __launch_bounds__(BLK) k_sums_final(Work w, const int *__restric
 
 // x1 fluxes: FP64 decomposition (psf in LDS, stored for the update), FP32
 // reconstruction of the perturbation and FP32 Riemann flux.
-__device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st, int lb,
+// FP64 series of the mixed form's decomposition (accurate to ~1e-15, far
+// below the FP32 perturbations they feed).  atanh(t)/t for |t| <= 0.3, exp(x)
+// for |x| <= 0.5, and ln((1 - exp(-l)) / l) from the Bernoulli-number series of
+// ln((e^x - 1)/x) = x/2 + x^2/24 - x^4/2880 + x^6/181440 - x^8/9676800 + ...
+__device__ __forceinline__ double atanh_over_t(double t2) {
+  double s = 1.0 / 21;
+#pragma unroll
+  for (int m = 19; m >= 3; m -= 2) s = fma(s, t2, 1.0 / m);
+  return fma(s, t2, 1.0);
+}
+
+__device__ __forceinline__ double exp_small(double x) {
+  double s = 1.0 / 6402373705728000.0;  // 1/18!
+  const double inv[17] = {1.0 / 355687428096000.0, 1.0 / 20922789888000.0,
+                          1.0 / 1307674368000.0,   1.0 / 87178291200.0,
+                          1.0 / 6227020800.0,      1.0 / 479001600.0,
+                          1.0 / 39916800.0,        1.0 / 3628800.0,
+                          1.0 / 362880.0,          1.0 / 40320.0,
+                          1.0 / 5040.0,            1.0 / 720.0,
+                          1.0 / 120.0,             1.0 / 24.0,
+                          1.0 / 6.0,               0.5,
+                          1.0};
+#pragma unroll
+  for (int m = 0; m < 17; ++m) s = fma(s, x, inv[m]);
+  return fma(s, x, 1.0);
+}
+
+__device__ __forceinline__ double ln_expm1_ratio(double l) {  // ln((1 - e^-l) / l)
+  double l2 = l * l;
+  double s = fma(fma(fma(-1.0 / 9676800.0, l2, 1.0 / 181440.0), l2, -1.0 / 2880.0), l2,
+                 1.0 / 24.0);
+  return fma(s, l2, -0.5 * l);
+}
+
+// x1 fluxes of the mixed form: the FP64 face pressure psf as the reference
+// computes it (psf in LDS and stored for the update), then the cell values
+// psv and rho_sv and the face values rho_sf of the decomposition from one
+// log-domain step per face: lambda = ln(a / b) = 2 atanh((a - b) / (a + b))
+// for the faces a, b of a cell gives psv = (a - b) / lambda,
+// rho_sf(b) = rho_sf(a) e^(-lambda / gamma) and
+// rho_sv = rho_sf(a) ((1 - e^-lambda) / lambda)^(1/gamma).  Outside the
+// series' range a cell falls back to the exact log / pow.
+__device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &st, int lb,
                                                  double *lds) {
 #pragma clang fp contract(fast)
   int lane = threadIdx.x;
@@ -1467,18 +1655,38 @@ __device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st,
   double ent0 = g.gamma;
   double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
   double ig = 1. / ent0, e1 = dexp(-ent1 / ent0);
+  // cell i from rho_sf at its lower face: psv, rho_sv and rho_sf at face i+1
+  auto decomp = [&](int i, double rsf_lo, double *psv, double *rsv) -> double {
+    double a = PSF(i), b = PSF(i + 1);
+    double t = (a - b) / (a + b);
+    if (fabs(t) > 0.3) {  // outside the series' range: the exact functions
+      double ps = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
+      *psv = ps;
+      *rsv = dpow(ps, ig) * e1;
+      return dpow(b, ig) * e1;
+    }
+    double lam = 2.0 * t * atanh_over_t(t * t);
+    *psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / lam;
+    *rsv = rsf_lo * exp_small(ig * ln_expm1_ratio(lam));
+    return rsf_lo * exp_small(-ig * lam);
+  };
   float q[6][5];
+  double rsf[4];  // rho_sf at the faces fi .. fi+3 of the window
+  double rlo = dpow(PSF(is - 3), ig) * e1;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
     int i = is - 3 + s;
-    double a = PSF(i), b = PSF(i + 1);
-    double psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
-    q[s][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - dpow(psv, ig) * e1);
+    double psv, rsv;
+    double rhi = decomp(i, rlo, &psv, &rsv);
+    if (s >= 3) rsf[s - 3] = rlo;  // faces is .. is+2
+    rlo = rhi;
+    q[s][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - rsv);
     q[s][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
     q[s][IVY] = (float)st.w[g.c(IVY, p, k, i, j)];
     q[s][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
     q[s][IPR] = (float)(st.w[g.c(IPR, p, k, i, j)] - psv);
   }
+  rsf[3] = rlo;  // face is+3
   float cth = (float)g.T(T_CTH1, p, k, j), cosc = (float)g.T(T_COSC, p, k, j);
   float gamma = (float)g.gamma;
   for (int fi = is; fi <= ie + 1; ++fi) {
@@ -1493,7 +1701,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st,
     // RestoreFromPerturbation clamps (change_to_perturbation.cpp:174-189)
     if (wl[IPR] < -P) wl[IPR] = 0.f;
     if (wr[IPR] < -P) wr[IPR] = 0.f;
-    float dsf = (float)(dpow(pf, ig) * e1);
+    float dsf = (float)rsf[0];
     wl[IDN] += dsf;
     if (wl[IDN] < 0.f) wl[IDN] = dsf;
     wr[IDN] += dsf;
@@ -1507,9 +1715,13 @@ __device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st,
 #pragma unroll
         for (int n = 0; n < 5; ++n) q[s][n] = q[s + 1][n];
       int i = fi + 3;
-      double a = PSF(i), b = PSF(i + 1);
-      double psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
-      q[5][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - dpow(psv, ig) * e1);
+      double psv, rsv;
+      double rhi = decomp(i, rsf[3], &psv, &rsv);
+      rsf[0] = rsf[1];
+      rsf[1] = rsf[2];
+      rsf[2] = rsf[3];
+      rsf[3] = rhi;
+      q[5][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - rsv);
       q[5][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
       q[5][IVY] = (float)st.w[g.c(IVY, p, k, i, j)];
       q[5][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
@@ -1528,7 +1740,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const Grid &g, const State &st,
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64) k_flux_x1_mix(Work w) {
+__global__ void __launch_bounds__(64) k_flux_x1_mix(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
@@ -1537,7 +1749,7 @@ __global__ void __launch_bounds__(64) k_flux_x1_mix(Work w) {
 }
 
 // cell value as an FP32 perturbation about the level reference
-__device__ __forceinline__ float pert(const Grid &g, const State &st, int n, int p, int k,
+__device__ __forceinline__ float pert(const DGrid &g, const DState &st, int n, int p, int k,
                                       int i, int j) {
   double v = st.w[g.c(n, p, k, i, j)];
   if (g.eos == EOS_SW) {
@@ -1549,14 +1761,14 @@ __device__ __forceinline__ float pert(const Grid &g, const State &st, int n, int
   return (float)v;
 }
 
-__device__ __forceinline__ void edge_mix_body(const Grid &g, const State &st, int lb,
-                                              const EdgeDesc *__restrict__ ed, int nedge) {
+__device__ __forceinline__ void edge_mix_body(const DGrid &g, const DState &st, int lb,
+                                              const ADDR(1) EdgeDesc *__restrict__ ed, int nedge) {
 #pragma clang fp contract(fast)
   int t = lb * BLK + threadIdx.x;
   int ni = g.nx1;
   if (t >= nedge * ni) return;
   int e = t / ni, i = g.is + t % ni;
-  EdgeDesc d = ed[e];
+  EdgeDesc d = *(const EdgeDesc *)(ed + e);
   int p = d.src_panel, k = d.src_k, j = d.src_j;
   float q[5] = {0, 0, 0, 0, 0};
 #pragma unroll
@@ -1593,15 +1805,15 @@ __device__ __forceinline__ void edge_mix_body(const Grid &g, const State &st, in
     if (n < g.nv) st.fedge[n * vs + base] = q[n];
 }
 
-__global__ void __launch_bounds__(BLK) k_edge_states_mix(Work w) {
+__global__ void __launch_bounds__(BLK) k_edge_states_mix(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    edge_mix_body(g, st, lb, w.inst[ii_].ed, w.inst[ii_].ned);
+    edge_mix_body(g, st, lb, glb(I_[ii_].ed), I_[ii_].ned);
   }
 }
 
 template <int dir>
-__device__ __forceinline__ void flux_h_mix_body(const Grid &g, const State &st, int lb) {
+__device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st, int lb) {
 #pragma clang fp contract(fast)
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
@@ -1672,7 +1884,7 @@ __device__ __forceinline__ void flux_h_mix_body(const Grid &g, const State &st, 
   }
   riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf,
                sinf, 0.f);
-  float *F = st.fflx[dir];
+  ADDR(1) float *F = st.fflx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
     if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
@@ -1688,7 +1900,7 @@ __device__ __forceinline__ void flux_h_mix_body(const Grid &g, const State &st, 
 }
 
 template <int dir>
-__global__ void __launch_bounds__(BLK) k_flux_h_mix(Work w) {
+__global__ void __launch_bounds__(BLK) k_flux_h_mix(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
     flux_h_mix_body<dir>(g, st, lb);
@@ -1741,7 +1953,7 @@ namespace {
 // launch kinds, each with its own block-prefix table over the instances
 enum {
   K_PRIM = 0, K_GHOST_U, K_GHOST_S, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX,
-  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_NKIND
+  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -1770,7 +1982,7 @@ struct Launcher {
   std::vector<int> ev_kind;
 
   Work work(int kind) const {
-    return Work{d_inst, d_bstart + (size_t)kind * (ninst + 1), ninst, nblk[kind]};
+    return Work{d_bstart + (size_t)kind * (ninst + 1), ninst, nblk[kind]};
   }
   unsigned grid(int kind) const {
     int n = nblk[kind];
@@ -1837,7 +2049,7 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, bool per_
   size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
   size_t cols = (size_t)g.N * g.N;
   switch (kind) {
-    case K_PRIM: return ((g.nc2 + 63) / 64) * g.nc3 * 6;
+    case K_PRIM: return (int)grid1((size_t)6 * g.nc3 * g.nx1 * g.nc2);
     case K_GHOST_U: return (int)grid1((size_t)ngd * g.nx1);
     case K_GHOST_S: return g.nt ? (int)grid1((size_t)ngd * g.nx1) : 0;
     case K_X1: return ideal && !mixed ? (int)grid1(cols, 64) * 6 : 0;
@@ -1854,6 +2066,7 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, bool per_
     case K_ADD_DU: return ideal && !g.implicit ? (int)grid1(cells) : 0;
     case K_CELLS: return (int)grid1(cells);
     case K_ONE: return 1;
+    case K_DT: return (int)grid1(cells, BLK * DTC);
   }
   return 0;
 }
@@ -1891,6 +2104,10 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     bool need_x1 = L.nblk[K_X1] > 0 || L.nblk[K_X1_MIX] > 0;
+    for (const exo_handle *h : hs)
+      if (h->g.nc1 > IMPL_MAXNC1 && h->g.eos == EXO_EOS_IDEAL && h->g.implicit)
+        throw std::string("nx1 too large for the implicit column solve (nc1 <= ") +
+            std::to_string(IMPL_MAXNC1) + ")";
     if (need_x1 && L.lds_x1 > (size_t)maxlds)
       throw std::string("nx1 too large: the x1 column kernel needs ") +
           std::to_string(L.lds_x1) + " bytes of LDS, the device allows " +
@@ -1911,15 +2128,15 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
 
 // one launch of a work-list kernel; kid < 0: not timed
 template <typename... P, typename... A>
-void launch(Launcher &L, int kind, int kid, void (*kern)(Work, P...), unsigned block, size_t lds,
-            A... args) {
+void launch(Launcher &L, int kind, int kid, void (*kern)(const Inst *, Work, P...), unsigned block,
+            size_t lds, A... args) {
   if (L.nblk[kind] == 0) return;
   bool t = L.prof && kid >= 0;
   if (t) {
     L.ev_kind.push_back(kid);
     HIPCHK(hipEventRecord(next_event(L), L.stream));
   }
-  kern<<<dim3(L.grid(kind)), dim3(block), lds, L.stream>>>(L.work(kind), args...);
+  kern<<<dim3(L.grid(kind)), dim3(block), lds, L.stream>>>(L.d_inst, L.work(kind), args...);
   if (t) HIPCHK(hipEventRecord(next_event(L), L.stream));
 }
 
@@ -2069,7 +2286,7 @@ void init_handle(exo_handle *h) {
 void launch_fill_ghosts(Launcher &L) {
   launch(L, K_GHOST_U, KID_GHOST, k_ghost, BLK, 0, 0);
   launch(L, K_GHOST_S, KID_GHOST, k_ghost, BLK, 0, 1);
-  launch(L, K_PRIM, KID_PRIM, k_prim, 64, 0, DFLOOR, SFLOOR);
+  launch(L, K_PRIM, KID_PRIM, k_prim, BLK, 0, DFLOOR, SFLOOR);
   HIPCHK(hipGetLastError());
 }
 
@@ -2108,7 +2325,7 @@ void launch_implicit(Launcher &L, double beta) {
 void launch_dt(Launcher &L, double dt_fixed) {
   if (dt_fixed <= 0.0) {
     launch(L, K_ONE, -1, k_dt_reset, 64, 0);
-    launch(L, K_CELLS, KID_DT, k_dt, BLK, 0);
+    launch(L, K_DT, KID_DT, k_dt, BLK, 0);
   }
   launch(L, K_ONE, -1, k_dt_finish, 64, 0, dt_fixed);
   HIPCHK(hipGetLastError());
@@ -2146,7 +2363,7 @@ void tune_launcher(Launcher &L, int reps) {
   HIPCHK(hipEventCreate(&e1));
   bool prof = L.prof;
   L.prof = false;
-  const int kinds[] = {K_PRIM, K_GHOST_U, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX, K_CELLS};
+  const int kinds[] = {K_PRIM, K_GHOST_U, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX, K_DT};
   for (int kind : kinds) {
     if (L.nblk[kind] == 0) continue;
     std::vector<int> cands = {0};
@@ -2159,7 +2376,7 @@ void tune_launcher(Launcher &L, int reps) {
       for (int r = -1; r < reps; ++r) {  // r = -1 warms up
         if (r == 0) HIPCHK(hipEventRecord(e0, L.stream));
         switch (kind) {
-          case K_PRIM: launch(L, K_PRIM, -1, k_prim, 64, 0, DFLOOR, SFLOOR); break;
+          case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
           case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
           case K_X1: launch(L, K_X1, -1, k_flux_x1, 64, L.lds_x1); break;
           case K_X1_MIX: launch(L, K_X1_MIX, -1, k_flux_x1_mix, 64, L.lds_x1); break;
@@ -2167,7 +2384,7 @@ void tune_launcher(Launcher &L, int reps) {
           case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
           case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
           case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1>, BLK, 0); break;
-          case K_CELLS: launch(L, K_CELLS, -1, k_dt, BLK, 0); break;
+          case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
         }
       }
       HIPCHK(hipEventRecord(e1, L.stream));
@@ -2181,9 +2398,9 @@ void tune_launcher(Launcher &L, int reps) {
     }
     L.cap[kind] = best_cap;
   }
-  // kinds sharing a shape with a tuned one
+  // kinds sharing a shape with a tuned one (update-shaped kinds keep the plain
+  // launch: they stream and cannot be repeated without changing the state)
   L.cap[K_GHOST_S] = L.cap[K_GHOST_U];
-  L.cap[K_UPD] = L.cap[K_UPD_MIX] = L.cap[K_ADD_DU] = L.cap[K_CELLS];
   L.prof = prof;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));
