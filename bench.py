@@ -64,6 +64,24 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
     }.get(kernel)
 
 
+# kernel of the live profile -> its name in the rocprofv3 PMC summary
+PMC_NAMES = {"implicit": "k_implicit", "update": "k_update<true>", "flux_x1": "k_flux_x1_mix",
+             "flux_x2": "k_flux_h_mix<1>", "flux_x3": "k_flux_h_mix<2>", "prim": "k_prim"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the mixed-form kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this workload (scaled by a
+    plain copy of known size); None if that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_exo3_96x96x64_lmars_implicit.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get(PMC_NAMES.get(kernel, ""))
+    except (OSError, ValueError):
+        return None
+    return k["hbm_bytes"] if k else None
+
+
 def kernel_rooflines(prof, cells, nv, mixed, nt=0):
     """per kernel: average launch time and achieved algorithmic GB/s"""
     out = {}
@@ -260,7 +278,19 @@ def main():
     ap.add_argument("--requests", type=int, default=16)
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc pass)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
@@ -290,7 +320,7 @@ def main():
     achieved = (bpc * r["cells"] / avg_s / 1e9) if bpc else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None, "avg_launch_ms": tot_ms / cnt,
+            "traffic": measured_traffic(dom), "avg_launch_ms": tot_ms / cnt,
             "bytes_per_cell": bpc}
     kernels = kernel_rooflines(prof, r["cells"], r["nv"], True)
     for k in kernels:
