@@ -347,8 +347,29 @@ __global__ void __launch_bounds__(BLK) k_ghost(const Inst *__restrict__ I_, Work
 // (change_to_perturbation.cpp:39-190) around Weno5X1, then the x1 Riemann
 // solve (patches/19.calculate_fluxes.patch:77-106).  The face pressure psf
 // lives in LDS, [face][lane].
+// The block holds 64 columns and X1W waves: wave 0 integrates the face
+// pressure of every column into LDS, then each wave takes a contiguous share
+// of the faces with its own WENO window, so a column's faces are worked on by
+// X1W lanes at once.  Per-face arithmetic is independent of the split.
+constexpr int X1W = 4;
+
+__device__ __forceinline__ void psf_scan(const DGrid &g, const DState &st, int p, int k, int j,
+                                         double *psf, int lane) {
+#define PSF(ii) psf[(ii) * 64 + lane]
+  int is = g.is, ie = g.ie;
+  double grav = -g.grav;
+  double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
+  double RdTv = pie / die;
+  PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * RdTv));
+  for (int i = ie; i >= is; --i)
+    PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
+  for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
+  for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
+#undef PSF
+}
+
 __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, int lb, double *lds) {
-  int lane = threadIdx.x;
+  int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
   int N = g.N;
   int nbx = (N * N + 63) / 64;
   int col = (lb % nbx) * 64 + lane;
@@ -359,22 +380,19 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
 #define PSF(ii) psf[(ii) * 64 + lane]
   int is = g.is, ie = g.ie, nv = g.nv;
   bool decomp = g.grav != 0.;
-  double grav = -g.grav;
   double e1 = 1., ig = 1.;
-  if (act && decomp) {
-    double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
-    double RdTv = pie / die;
-    PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * RdTv));
-    for (int i = ie; i >= is; --i)
-      PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
-    for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
-    for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
+  if (seg == 0 && act && decomp) psf_scan(g, st, p, k, j, psf, lane);
+  __syncthreads();
+  if (!act) return;
+  if (decomp) {
     double ent0 = g.gamma;
     double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
     ig = 1. / ent0;
     e1 = dexp(-ent1 / ent0);
   }
-  if (!act) return;
+  // this wave's faces [f0, f1)
+  int nf = ie + 2 - is;
+  int f0 = is + (nf * seg) / X1W, f1 = is + (nf * (seg + 1)) / X1W;
   // window of 6 cells (f-3 .. f+2) for face f, perturbed IDN / IPR
   double q[6][5];
   auto load = [&](int slot, int i) {
@@ -390,9 +408,9 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
     }
   };
 #pragma unroll
-  for (int s = 0; s < 6; ++s) load(s, is - 3 + s);
+  for (int s = 0; s < 6; ++s) load(s, f0 - 3 + s);
   double cth = g.T(T_CTH1, p, k, j), cosc = g.T(T_COSC, p, k, j);
-  for (int fi = is; fi <= ie + 1; ++fi) {
+  for (int fi = f0; fi < f1; ++fi) {
     double wl[5], wr[5], fl[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
@@ -417,7 +435,7 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
 #pragma unroll
     for (int n = 0; n < 5; ++n)
       if (n < nv) st.flx[0][g.f(n, p, k, fi, j)] = fl[n];
-    if (fi <= ie) {
+    if (fi + 1 < f1) {
 #pragma unroll
       for (int s = 0; s < 5; ++s)
 #pragma unroll
@@ -427,7 +445,7 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
   }
   // passive scalars: WENO of r along x1, upwinded by the mass flux
   for (int n = 0; n < g.nt; ++n)
-    for (int fi = is; fi <= ie + 1; ++fi) {
+    for (int fi = f0; fi < f1; ++fi) {
       double r0 = st.r[g.c(n, p, k, fi - 3, j)], r1 = st.r[g.c(n, p, k, fi - 2, j)];
       double r2 = st.r[g.c(n, p, k, fi - 1, j)], r3 = st.r[g.c(n, p, k, fi, j)];
       double r4 = st.r[g.c(n, p, k, fi + 1, j)], r5 = st.r[g.c(n, p, k, fi + 2, j)];
@@ -439,11 +457,12 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64) k_flux_x1(const Inst *__restrict__ I_, Work w) {
+__global__ void __launch_bounds__(64 * X1W) k_flux_x1(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
     flux_x1_body(g, st, lb, lds);
+    __syncthreads();  // psf is rebuilt by the next block of the loop
   }
 }
 
@@ -1635,23 +1654,21 @@ __device__ __forceinline__ double ln_expm1_ratio(double l) {  // ln((1 - e^-l) /
 __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &st, int lb,
                                                  double *lds) {
 #pragma clang fp contract(fast)
-  int lane = threadIdx.x;
+  int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
   int N = g.N;
   int nbx = (N * N + 63) / 64;
   int col = (lb % nbx) * 64 + lane;
   int p = lb / nbx;
-  if (col >= N * N) return;
-  int k = g.ks + col / N, j = g.js + col % N;
+  bool act = col < N * N;
+  int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
   double *psf = lds;
 #define PSF(ii) psf[(ii) * 64 + lane]
   int is = g.is, ie = g.ie;
-  double grav = -g.grav;
-  double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
-  PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * (pie / die)));
-  for (int i = ie; i >= is; --i)
-    PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
-  for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
-  for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
+  if (seg == 0 && act) psf_scan(g, st, p, k, j, psf, lane);
+  __syncthreads();
+  if (!act) return;
+  int nf = ie + 2 - is;
+  int f0 = is + (nf * seg) / X1W, f1 = is + (nf * (seg + 1)) / X1W;
   double ent0 = g.gamma;
   double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
   double ig = 1. / ent0, e1 = dexp(-ent1 / ent0);
@@ -1672,13 +1689,13 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
   };
   float q[6][5];
   double rsf[4];  // rho_sf at the faces fi .. fi+3 of the window
-  double rlo = dpow(PSF(is - 3), ig) * e1;
+  double rlo = dpow(PSF(f0 - 3), ig) * e1;
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
-    int i = is - 3 + s;
+    int i = f0 - 3 + s;
     double psv, rsv;
     double rhi = decomp(i, rlo, &psv, &rsv);
-    if (s >= 3) rsf[s - 3] = rlo;  // faces is .. is+2
+    if (s >= 3) rsf[s - 3] = rlo;  // faces f0 .. f0+2
     rlo = rhi;
     q[s][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - rsv);
     q[s][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
@@ -1686,10 +1703,10 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
     q[s][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
     q[s][IPR] = (float)(st.w[g.c(IPR, p, k, i, j)] - psv);
   }
-  rsf[3] = rlo;  // face is+3
+  rsf[3] = rlo;  // face f0+3
   float cth = (float)g.T(T_CTH1, p, k, j), cosc = (float)g.T(T_COSC, p, k, j);
   float gamma = (float)g.gamma;
-  for (int fi = is; fi <= ie + 1; ++fi) {
+  for (int fi = f0; fi < f1; ++fi) {
     float wl[5], wr[5], fl[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
@@ -1709,7 +1726,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
     riemann_pert(g.eos, g.rs, IVX, gamma, wl, wr, P, fl, cth, 0.f, 1.f, cosc);
     for (int n = 0; n < 5; ++n) st.fflx[0][g.f(n, p, k, fi, j)] = fl[n];
     st.psf[g.f(0, p, k, fi, j)] = pf;
-    if (fi <= ie) {
+    if (fi + 1 < f1) {
 #pragma unroll
       for (int s = 0; s < 5; ++s)
 #pragma unroll
@@ -1729,7 +1746,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
     }
   }
   for (int n = 0; n < g.nt; ++n)
-    for (int fi = is; fi <= ie + 1; ++fi) {
+    for (int fi = f0; fi < f1; ++fi) {
       float r[6];
       for (int s = 0; s < 6; ++s) r[s] = (float)st.r[g.c(n, p, k, fi - 3 + s, j)];
       float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
@@ -1740,11 +1757,12 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64) k_flux_x1_mix(const Inst *__restrict__ I_, Work w) {
+__global__ void __launch_bounds__(64 * X1W) k_flux_x1_mix(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
     flux_x1_mix_body(g, st, lb, lds);
+    __syncthreads();  // psf is rebuilt by the next block of the loop
   }
 }
 
@@ -2291,8 +2309,8 @@ void launch_fill_ghosts(Launcher &L) {
 }
 
 void launch_fluxes(Launcher &L) {
-  launch(L, K_X1, KID_FLUX_X1, k_flux_x1, 64, L.lds_x1);
-  launch(L, K_X1_MIX, KID_FLUX_X1, k_flux_x1_mix, 64, L.lds_x1);
+  launch(L, K_X1, KID_FLUX_X1, k_flux_x1, 64 * X1W, L.lds_x1);
+  launch(L, K_X1_MIX, KID_FLUX_X1, k_flux_x1_mix, 64 * X1W, L.lds_x1);
   launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
   launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
@@ -2378,8 +2396,8 @@ void tune_launcher(Launcher &L, int reps) {
         switch (kind) {
           case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
           case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
-          case K_X1: launch(L, K_X1, -1, k_flux_x1, 64, L.lds_x1); break;
-          case K_X1_MIX: launch(L, K_X1_MIX, -1, k_flux_x1_mix, 64, L.lds_x1); break;
+          case K_X1: launch(L, K_X1, -1, k_flux_x1, 64 * X1W, L.lds_x1); break;
+          case K_X1_MIX: launch(L, K_X1_MIX, -1, k_flux_x1_mix, 64 * X1W, L.lds_x1); break;
           case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
           case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
           case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
