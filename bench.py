@@ -241,7 +241,7 @@ def serving(ex, precision, nreq, keep=2):
                              x1min=R_EARTH, x1max=R_EARTH + 30000.0, gamma=1.4, grav=-9.81,
                              omega=7.292e-5, cfl=0.4, precision=precision)
         reqs.append((cfg, (T0, 1.0e5, 287.0, 0.01, 5.0, 3.0e-3, 0.5)))
-    srv = ex.Server(max_active=16, chunk_steps=20, graph=True, prep_threads=4)
+    srv = ex.Server(max_active=4, chunk_steps=20, graph=True, prep_threads=4)
     ids = []
     t0 = time.perf_counter()
     for q, (cfg, par) in enumerate(reqs):

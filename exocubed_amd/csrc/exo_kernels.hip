@@ -65,6 +65,10 @@ struct GridT {
   __host__ __device__ size_t c(int v, int p, int k, int i, int j) const {
     return v * vstride + p * pstride + k * plane + (size_t)i * jp + j;
   }
+  // variables a kernel can skip: the shallow-water x1 velocity is zero for
+  // good (shallow_yz_hydro.cpp:57 sets it, every flux of it is zero), so it is
+  // never reconstructed, fluxed, updated or stored after the first fill
+  __host__ __device__ bool skipv(int n) const { return n >= nv || (eos == 0 && n == 1); }
   __host__ __device__ size_t f(int v, int p, int k, int i, int j) const {
     return v * fvstride + p * fpstride + k * fplane + (size_t)i * jp + j;
   }
@@ -256,14 +260,16 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
     wv[IVZ] = w_vz;
     wv[IPR] = gm1 * (u_e - KE) * 1. / 1.;
   }
-  int qb = i - g.is + 1, qt = g.ie - i + 1;  // mirrors into ghosts is - qb, ie + qt
+  // mirrors into ghosts is - qb, ie + qt (shallow water never reads x1 ghosts)
+  int qb = i - g.is + 1, qt = g.ie - i + 1;
+  bool mirror = g.eos != EOS_SW;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     st.w[g.c(n, p, k, i, j)] = wv[n];
     double m = n == IVX ? -wv[n] : wv[n];
-    if (qb <= NG) st.w[g.c(n, p, k, g.is - qb, j)] = m;
-    if (qt <= NG) st.w[g.c(n, p, k, g.ie + qt, j)] = m;
+    if (mirror && qb <= NG) st.w[g.c(n, p, k, g.is - qb, j)] = m;
+    if (mirror && qt <= NG) st.w[g.c(n, p, k, g.ie + qt, j)] = m;
   }
   for (int n = 0; n < g.nt; ++n) {
     double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / st.u[g.c(IDN, p, k, i, j)]);
@@ -304,7 +310,7 @@ __device__ __forceinline__ void ghost_body(const DGrid &g, int lb,
   }
   size_t ot = g.c(0, d.tgt_panel, d.tgt_k, i, d.tgt_j);
   for (int n = 0; n < nvar; ++n) {
-    if (vectors && (n == IVY || n == IVZ)) continue;
+    if (vectors && (n == IVY || n == IVZ || g.skipv(n))) continue;
     double v1 = arr[n * g.vstride + o0], v2 = arr[n * g.vstride + o1];
     arr[n * g.vstride + ot] = (d.A * v1 + d.B * v2) / d.D;
   }
@@ -482,7 +488,7 @@ __device__ __forceinline__ void edge_body(const DGrid &g, const DState &st, int 
   double q[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     double v;
     if (d.src_dir == 0)  // R state of cell js at face js
       v = weno5(st.w[g.c(n, p, k, i, j - 2)], st.w[g.c(n, p, k, i, j - 1)],
@@ -511,7 +517,7 @@ __device__ __forceinline__ void edge_body(const DGrid &g, const DState &st, int 
   size_t vs = (size_t)6 * 4 * g.N * ni;
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (n < g.nv) st.edge[n * vs + base] = q[n];
+    if (!g.skipv(n)) st.edge[n * vs + base] = q[n];
 }
 
 __global__ void __launch_bounds__(BLK) k_edge_states(const Inst *__restrict__ I_, Work w) {
@@ -555,7 +561,7 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
   double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     double m3, m2, m1, c0, p1, p2;
     if (dir == 1) {
       m3 = st.w[g.c(n, p, k, i, j - 3)];
@@ -581,12 +587,12 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
     size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (n < g.nv) wl[n] = st.edge[n * vs + b];
+      if (!g.skipv(n)) wl[n] = st.edge[n * vs + b];
   } else if (fa == N) {  // ghost side is the R state
     size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (n < g.nv) wr[n] = st.edge[n * vs + b];
+      if (!g.skipv(n)) wr[n] = st.edge[n * vs + b];
   }
   double cosf, sinf, cth;
   if (dir == 1) {
@@ -602,7 +608,7 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
   ADDR(1) double *F = st.flx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
+    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
   for (int n = 0; n < g.nt; ++n) {
     double r[6];
     for (int s = 0; s < 6; ++s)
@@ -649,7 +655,11 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   double wght = sw.beta * dt;
   bool tend = g.eos == EOS_IDEAL;
   double out[5];
-  double a1l = g.area1(p, k, j, i), a1r = g.area1(p, k, j, i + 1);
+  double a1l = 0., a1r = 0.;  // x1 faces: shallow water has no x1 flux
+  if (g.eos != EOS_SW) {
+    a1l = g.area1(p, k, j, i);
+    a1r = g.area1(p, k, j, i + 1);
+  }
   double a2l = g.area2(p, k, j, i), a2r = g.area2(p, k, j + 1, i);
   double a3l = g.area3(p, k, j, i), a3r = g.area3(p, k + 1, j, i);
   double vol = g.volume(p, k, j, i);
@@ -659,7 +669,7 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   bool keep_u = sw.first && sw.delta == 1.0;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     size_t c = g.c(n, p, k, i, j);
     double u = st.u[c], u1;
     if (sw.first) {
@@ -765,7 +775,8 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
     }
   }
   ADDR(1) double *dst = tend ? st.du : st.u;
-  for (int n = 0; n < g.nv; ++n) dst[g.c(n, p, k, i, j)] = out[n];
+  for (int n = 0; n < g.nv; ++n)
+    if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
   // passive scalars: weighted average and flux divergence into s
   for (int n = 0; n < g.nt; ++n) {
     size_t c = g.c(n, p, k, i, j);
@@ -1436,7 +1447,7 @@ __device__ __forceinline__ void dt_body(const DGrid &g, const DState &st, int lb
     double wi[5] = {0., 0., 0., 0., 0.};
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (n < g.nv) wi[n] = st.w[g.c(n, p, k, i, j)];
+      if (!g.skipv(n)) wi[n] = st.w[g.c(n, p, k, i, j)];
     double cs = g.eos == EOS_SW ? sqrt(wi[IDN])
                                 : sqrt((1. + (g.gamma - 1) * 1. / 1.) * wi[IPR] / wi[IDN]);
     double dt1 = g.dx1f[i];
@@ -1791,7 +1802,7 @@ __device__ __forceinline__ void edge_mix_body(const DGrid &g, const DState &st, 
   float q[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     float v;
     if (d.src_dir == 0)
       v = weno5f(pert(g, st, n, p, k, i, j - 2), pert(g, st, n, p, k, i, j - 1),
@@ -1820,7 +1831,7 @@ __device__ __forceinline__ void edge_mix_body(const DGrid &g, const DState &st, 
   size_t vs = (size_t)6 * 4 * g.N * ni;
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (n < g.nv) st.fedge[n * vs + base] = q[n];
+    if (!g.skipv(n)) st.fedge[n * vs + base] = q[n];
 }
 
 __global__ void __launch_bounds__(BLK) k_edge_states_mix(const Inst *__restrict__ I_, Work w) {
@@ -1859,7 +1870,7 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (n >= g.nv) continue;
+    if (g.skipv(n)) continue;
     float v[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s)
@@ -1873,12 +1884,12 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
     size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (n < g.nv) wl[n] = st.fedge[n * vs + b];
+      if (!g.skipv(n)) wl[n] = st.fedge[n * vs + b];
   } else if (fa == N) {
     size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (n < g.nv) wr[n] = st.fedge[n * vs + b];
+      if (!g.skipv(n)) wr[n] = st.fedge[n * vs + b];
   }
   // full density for the ideal gas; shallow water keeps h' (the solver adds H)
   float P;
@@ -1905,7 +1916,7 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   ADDR(1) float *F = st.fflx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (n < g.nv) F[g.f(n, p, k, i, j)] = fl[n];
+    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
   for (int n = 0; n < g.nt; ++n) {
     float r[6];
     for (int s = 0; s < 6; ++s)
