@@ -115,13 +115,15 @@ def run_gpu(ex, precision, steps, warmup, perturb=0.0):
     if warmup:
         g.step(warmup)
     g.sync()
-    g.profile_begin()
     t0 = time.perf_counter()
-    dt = g.step(steps)
+    dt = g.step(steps)  # each step replayed from a captured HIP graph
     g.sync()
     t1 = time.perf_counter()
-    prof = g.profile_end()
     u, _ = g.download()
+    # per-kernel times: a further short run with a HIP event pair per launch
+    g.profile_begin()
+    g.step(min(steps, 20))
+    prof = g.profile_end()
     finite = bool(np.isfinite(u).all())
     g.close()
     sec = t1 - t0

@@ -2035,6 +2035,10 @@ struct exo_handle {
   double *d_sums = nullptr;     // [5] totals of exo_conserved_sums
   bool have_state = false;
   bool mixed = false;
+  // one SSPRK3 step captured as a HIP graph and replayed by exo_step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double graph_dt_fixed = 0.0;
 
   void *dalloc(size_t bytes) {
     void *p = nullptr;
@@ -2585,6 +2589,8 @@ int exo_create(const exo_config *cfg, exo_handle **out) {
 int exo_destroy(exo_handle *h) {
   if (!h) return EXO_ERR_ARG;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
   for (hipEvent_t e : h->L.ev_pool) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -2705,11 +2711,38 @@ int exo_stage(exo_handle *h, int stage, double dt) {
   });
 }
 
+namespace {
+// the step graph of a handle (captured on first use, rebuilt when the dt mode
+// or the launch widths change); kernel nodes only
+void handle_graph(exo_handle *h, double dt_fixed) {
+  if (h->exec && h->graph_dt_fixed == dt_fixed) return;
+  if (h->exec) (void)hipGraphExecDestroy(h->exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->exec = nullptr;
+  h->graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    run_step(h->L, dt_fixed);
+  } catch (...) {
+    hipGraph_t gr = nullptr;
+    (void)hipStreamEndCapture(h->stream, &gr);
+    if (gr) (void)hipGraphDestroy(gr);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(h->stream, &h->graph));
+  HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
+  h->graph_dt_fixed = dt_fixed;
+}
+}  // namespace
+
 int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
   return guarded(h, [&] {
     if (!h->have_state) throw std::string("no state uploaded");
-    for (int s = 0; s < nsteps; ++s) {
-      run_step(h->L, dt_fixed);
+    if (nsteps > 1 && !h->L.prof) {  // replay a captured step (no per-kernel timing)
+      handle_graph(h, dt_fixed);
+      for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(h->exec, h->stream));
+    } else {
+      for (int s = 0; s < nsteps; ++s) run_step(h->L, dt_fixed);
     }
     if (dt_last) {
       double v[2];
@@ -2865,6 +2898,10 @@ int exo_tune(exo_handle *h, int reps, int *caps, int ncaps) {
   return guarded(h, [&] {
     if (!h->have_state) throw std::string("no state uploaded");
     tune_launcher(h->L, reps > 0 ? reps : 3);
+    if (h->exec) (void)hipGraphExecDestroy(h->exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->exec = nullptr;
+    h->graph = nullptr;
     for (int q = 0; caps && q < ncaps && q < K_NKIND; ++q) caps[q] = h->L.cap[q];
   });
 }

@@ -26,6 +26,11 @@
 extern "C" {
 #endif
 
+/* the library is built with hidden visibility; only these entry points export */
+#ifndef EXO_API
+#define EXO_API __attribute__((visibility("default")))
+#endif
+
 #define EXO_OK 0
 #define EXO_ERR_ARG (-1)
 #define EXO_ERR_HIP (-2)
@@ -67,62 +72,62 @@ typedef struct exo_handle exo_handle;
  * (src/exo3/gnomonic_equiangle.cpp:24-221), CubedSphere (cubed_sphere.cpp:25-35),
  * Decomposition (decomposition.cpp:25-48) and ImplicitSolver
  * (implicit_solver.cpp:34-90) for all six panels at once. */
-int exo_create(const exo_config *cfg, exo_handle **out);
-int exo_destroy(exo_handle *h);
-const char *exo_last_error(const exo_handle *h);
+EXO_API int exo_create(const exo_config *cfg, exo_handle **out);
+EXO_API int exo_destroy(exo_handle *h);
+EXO_API const char *exo_last_error(const exo_handle *h);
 /* number of conserved variables (NHYDRO) and padded cells per panel */
-int exo_nvar(const exo_handle *h);
-size_t exo_state_doubles(const exo_handle *h); /* 6*NVAR*nc3*nc2*nc1 */
+EXO_API int exo_nvar(const exo_handle *h);
+EXO_API size_t exo_state_doubles(const exo_handle *h); /* 6*NVAR*nc3*nc2*nc1 */
 
 /* --- state transfer (host layout above) -------------------------------- */
 /* Upload conserved variables u and (if ntracer > 0) conserved scalars s = rho r
  * (NTRACER, nc3, nc2, nc1 per panel; may be NULL).  Interior cells are read;
  * ghosts are then filled as Mesh::Initialize does on the cube (ghost exchange,
  * patches/29.mesh.patch:19-86, then cons->prim and the reflecting x1 BC). */
-int exo_upload_state(exo_handle *h, const double *u, const double *s);
-int exo_download_state(exo_handle *h, double *u, double *s);
-int exo_download_prim(exo_handle *h, double *w);
+EXO_API int exo_upload_state(exo_handle *h, const double *u, const double *s);
+EXO_API int exo_download_state(exo_handle *h, double *u, double *s);
+EXO_API int exo_download_prim(exo_handle *h, double *w);
 
 /* --- the hot path, one call per reference entry point ------------------- */
 /* Hydro::CalculateFluxes (patches/19.calculate_fluxes.patch:17-163): WENO5
  * reconstruction (Reconstruction::Weno5X1/X2/X3), hydrostatic decomposition
  * around the x1 sweep (change_to_perturbation.cpp:39-190), panel-edge L/R
  * exchange (cubed_sphere.cpp:267-687) and Hydro::RiemannSolver on every face. */
-int exo_calc_fluxes(exo_handle *h);
+EXO_API int exo_calc_fluxes(exo_handle *h);
 /* Copy the face fluxes of direction dir (0 = x1, 1 = x2, 2 = x3) to the host in
  * Athena's flux layout: x1 (NVAR, nc3, nc2, nc1+1), x2 (NVAR, nc3, nc2+1, nc1),
  * x3 (NVAR, nc3+1, nc2, nc1) per panel; only interior faces are defined. */
-int exo_download_flux(exo_handle *h, int dir, double *out);
+EXO_API int exo_download_flux(exo_handle *h, int dir, double *out);
 /* Hydro::AddFluxDivergence(wght, u) followed by
  * GnomonicEquiangle::AddCoordTermsDivergence (gnomonic_equiangle.cpp:834-879)
  * into u (shallow water) or into the tendency du (ideal gas, zeroed first, as
  * ImplicitHydroTasks::IntegrateHydro, implicit_hydro_tasks.cpp:157-162). */
-int exo_add_flux_divergence(exo_handle *h, double wght);
+EXO_API int exo_add_flux_divergence(exo_handle *h, double wght);
 /* CubedSphereUtility::PackData panel-edge remap (cubed_sphere_utility.cpp:
  * 371-617) of the conserved state into the face ghost zones, then
  * cons->prim (EquationOfState::ConservedToPrimitive) and the reflecting x1 BC. */
-int exo_fill_ghosts(exo_handle *h);
+EXO_API int exo_fill_ghosts(exo_handle *h);
 /* ImplicitSolver::SolveImplicit3D (solve_implicit_3d.cpp:27-180) with
  * FullCorrection on the x1 columns, applied to du, then u += du
  * (ImplicitHydroTasks::ImplicitCorrection, implicit_hydro_tasks.cpp:245-266). */
-int exo_implicit_correct(exo_handle *h, double dt_stage);
+EXO_API int exo_implicit_correct(exo_handle *h, double dt_stage);
 /* Hydro::NewBlockTimeStep (patches/21.new_blockdt.patch:16-37) and
  * Mesh::NewTimeStep's factor-2 growth limit; returns the next dt. */
-int exo_new_dt(exo_handle *h, double *dt);
+EXO_API int exo_new_dt(exo_handle *h, double *dt);
 /* One SSPRK3 stage (1..3) with time step dt, as the task list runs it. */
-int exo_stage(exo_handle *h, int stage, double dt);
+EXO_API int exo_stage(exo_handle *h, int stage, double dt);
 /* nsteps full SSPRK3 steps, dt from the CFL condition on the device when
  * dt_fixed <= 0; *dt_last (may be NULL) receives the last step's dt.
  * Replaces the main loop body (main.cpp.in:130-186). */
-int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last);
+EXO_API int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last);
 /* Block until all work queued on the handle's stream has finished. */
-int exo_sync(exo_handle *h);
+EXO_API int exo_sync(exo_handle *h);
 /* Live kernel timing: between begin and end every launch on the handle's stream
  * is bracketed by a HIP event pair; end returns the summed milliseconds and
  * launch counts per kernel id (0 prim, 1 ghost, 2 flux_x1, 3 edge_states,
  * 4 flux_x2, 5 flux_x3, 6 update, 7 implicit, 8 dt). */
-int exo_profile_begin(exo_handle *h);
-int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
+EXO_API int exo_profile_begin(exo_handle *h);
+EXO_API int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
 
 /* --- conserved totals and launch tuning ---------------------------------- */
 /* Sum over the interior cells of u_n * CellVolume for every conserved
@@ -130,17 +135,17 @@ int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
  * (TwoSum) addition in a fixed order: deterministic, accurate to about one
  * rounding of each total.  The reference has no such diagnostic; it checks the
  * conservation of the finite-volume update (mass, and energy without sources). */
-int exo_conserved_sums(exo_handle *h, double *out5);
+EXO_API int exo_conserved_sums(exo_handle *h, double *out5);
 /* Autotune the persistent launch width of the handle's kernels that only read
  * the conserved state (repeated launches change nothing): each candidate
  * width is timed `reps` times and the fastest kept.  caps (may be NULL)
  * receives up to ncaps tuned widths, one per launch kind (0 = one workgroup
  * per block). */
-int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
+EXO_API int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
 
 /* Device bytes exo_create allocates for this configuration (host only; 0 for
  * an invalid configuration).  Used to admit work by memory footprint. */
-size_t exo_footprint(const exo_config *cfg);
+EXO_API size_t exo_footprint(const exo_config *cfg);
 
 /* --- batches of independent instances ----------------------------------- */
 /* A batch advances several handles (uneven sizes, any configuration) with one
@@ -154,22 +159,22 @@ size_t exo_footprint(const exo_config *cfg);
  * process (main.cpp.in:130-186). */
 typedef struct exo_batch exo_batch;
 #define EXO_BATCH_GRAPH 1
-int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **out);
-int exo_batch_destroy(exo_batch *b);
-const char *exo_batch_last_error(const exo_batch *b);
+EXO_API int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **out);
+EXO_API int exo_batch_destroy(exo_batch *b);
+EXO_API const char *exo_batch_last_error(const exo_batch *b);
 /* nsteps SSPRK3 steps of every member (dt from each member's CFL when
  * dt_fixed <= 0); returns when the work has finished */
-int exo_batch_step(exo_batch *b, int nsteps, double dt_fixed);
+EXO_API int exo_batch_step(exo_batch *b, int nsteps, double dt_fixed);
 /* exo_conserved_sums of every member: out[member][5] */
-int exo_batch_sums(exo_batch *b, double *out);
+EXO_API int exo_batch_sums(exo_batch *b, double *out);
 /* exo_tune over the batch's launches; exo_batch_set_caps sets widths directly */
-int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps);
-int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps);
+EXO_API int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps);
+EXO_API int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps);
 /* live kernel timing of the batch's launches (kernel ids as exo_profile_end;
  * graphs are not used while profiling) */
-int exo_batch_profile_begin(exo_batch *b);
-int exo_batch_profile_end(exo_batch *b, double *ms, int *counts, int nkinds);
-int exo_batch_sync(exo_batch *b);
+EXO_API int exo_batch_profile_begin(exo_batch *b);
+EXO_API int exo_batch_profile_end(exo_batch *b, double *ms, int *counts, int nkinds);
+EXO_API int exo_batch_sync(exo_batch *b);
 
 /* --- serving: continuous batching of forward-run requests ---------------- */
 /* A request loop in the library: prep threads build initial states, one
@@ -182,29 +187,29 @@ int exo_batch_sync(exo_batch *b);
  * free their memory at once.  A request's result does not depend on the mix.
  * No reference counterpart: the reference runs one instance per process. */
 typedef struct exo_server exo_server;
-int exo_server_create(size_t mem_budget, int max_active, int chunk_steps, int flags,
+EXO_API int exo_server_create(size_t mem_budget, int max_active, int chunk_steps, int flags,
                       int prep_threads, exo_server **out);
 /* Queue a forward run of nsteps SSPRK3 steps (CFL dt).  The initial state is
  * copied from u0/s0 (host layout above), or, with u0 == NULL, built by
  * exo_init_state_cfg(cfg, kind, seed, params).  keep_state = 0 drops the final
  * state (only the info of exo_server_wait is kept).  Fails with EXO_ERR_NOMEM
  * if the request can never fit the budget. */
-int exo_server_submit(exo_server *s, const exo_config *cfg, const double *u0, const double *s0,
+EXO_API int exo_server_submit(exo_server *s, const exo_config *cfg, const double *u0, const double *s0,
                       int kind, uint64_t seed, const double *params, int nparams, int nsteps,
                       int keep_state, uint64_t *id_out);
 /* Wait (timeout_s <= 0: forever) for a request, copy its final state to
  * u_out/s_out (may be NULL) and release it.  info[10] (may be NULL): latency,
  * queue wait, run time (s), last dt, cells, steps, mass before/after, energy
  * (shallow water: x2 momentum) total before/after. */
-int exo_server_wait(exo_server *s, uint64_t id, double timeout_s, double *u_out, double *s_out,
+EXO_API int exo_server_wait(exo_server *s, uint64_t id, double timeout_s, double *u_out, double *s_out,
                     double *info);
 /* stats[9]: completed, cell-updates, seconds first submit to last completion,
  * seconds in batch steps, batches formed, chunks, peak active, peak bytes,
  * budget bytes */
-int exo_server_stats(exo_server *s, double *stats, int n);
-const char *exo_server_last_error(const exo_server *s);
+EXO_API int exo_server_stats(exo_server *s, double *stats, int n);
+EXO_API const char *exo_server_last_error(const exo_server *s);
 /* Finishes the requests already on the device, drops queued ones. */
-int exo_server_destroy(exo_server *s);
+EXO_API int exo_server_destroy(exo_server *s);
 
 /* --- problem set-up (synthetic initial states of the benchmark configs) - */
 /* Fill host conserved arrays (interior cells) for a seeded initial state.
@@ -215,29 +220,29 @@ int exo_server_destroy(exo_server *s);
  *         wave amplitude);
  * kind 2: lapse-rate atmosphere to an isothermal stratosphere (params: Ts,
  *         lapse K/m, Tstrat, ps, Rd, noise K). */
-int exo_init_state(exo_handle *h, int kind, uint64_t seed, const double *params,
+EXO_API int exo_init_state(exo_handle *h, int kind, uint64_t seed, const double *params,
                    int nparams, double *u_out, double *s_out);
 /* The same without a handle (host only, no GPU needed). */
-int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, const double *params,
+EXO_API int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, const double *params,
                        int nparams, double *u_out, double *s_out);
 
 /* --- single-face and single-column shims for per-function parity ------- */
 /* Riemann flux of one face with the given L/R primitive states, computed by
  * the device kernel (dir 0/1/2 = x1/x2/x3; the metric of face (p,k,j) of this
  * handle's grid is used). */
-int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *wl,
+EXO_API int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *wl,
                      const double *wr, double *flx);
 /* WENO5 value at the face between b and c seen from cell c (device). */
-int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
+EXO_API int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
 /* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y), or
  * (op 3) the shared-reciprocal division x[2t] / x[2t+1] the implicit solve uses */
-int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n);
+EXO_API int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n);
 
 /* Host-only introspection of the set-up tables (no GPU needed): which 0 ghost
  * remap descriptors, 1 edge-exchange descriptors, 2 metric tables
  * [20][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f and 5 x3f of the six panels.
  * With buf == NULL only *count is returned. */
-int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap_bytes,
+EXO_API int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap_bytes,
                  size_t *count);
 
 #ifdef __cplusplus

@@ -4,6 +4,7 @@ set-up tables, argument checks) work without a GPU.  No kernel is launched."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -16,8 +17,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     txt = open(os.path.join(ROOT, "include", "exocubed.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(exo_[a-z0-9_]+)\s*\(",
-                       txt, flags=re.M)
+    names = re.findall(r"^\s*(?:EXO_API\s+)?(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*"
+                       r"(exo_[a-z0-9_]+)\s*\(", txt, flags=re.M)
     return sorted(set(names))
 
 
@@ -34,6 +35,13 @@ def test_library_exports_every_declared_symbol(ex):
     L = C.CDLL(ex.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(L, n)]
     assert not missing, missing
+
+
+def test_library_exports_nothing_else(ex):
+    out = subprocess.run(["nm", "-D", "--defined-only", ex.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert exported == set(declared_functions()), exported ^ set(declared_functions())
 
 
 def test_binding_covers_every_declared_symbol(ex):
