@@ -412,10 +412,20 @@ static double calc_interp_loc(int loc_n, int N_blk, int k, int ghost) {
 /* Interpolate 3 source rows along the edge (x2 = along j for x3 edges, or
  * x3 = along k for x2 edges), cubed_sphere_utility.cpp:92-221 (InteprolateX2)
  * and :223-369 (InteprolateX3).  src is an (nv, nc3, nc2, nc1) array; the
- * result tgt has shape (nv, ek-sk+1, ej-sj+1, ei-si+1). */
+ * result tgt has shape (nv, ek-sk+1, ej-sj+1, ei-si+1).
+ *
+ * `vectors` is 1 for the hydro set, whose IVY/IVZ pair is a covariant vector
+ * rotated into the target panel's basis, and 0 for passive scalars, which are
+ * interpolated as scalars.  The reference tests `n == IVY || n == IVZ` for any
+ * array it packs (:155, :284), scalars included (patches/28.bvals_cc.patch:21
+ * packs them with the same routine); with three scalars that reads past the
+ * scalar array, with four it rotates scalars 2 and 3 as a vector.  Restating
+ * it for scalars would pin undefined behaviour, so scalars stay scalars here,
+ * which equals the reference for up to two scalars. */
 static void interpolate_edge(const or_solver *s, const or_panel *P, const double *src,
-                             int nv, double *tgt, int along_x2, int DirInv, int TgtSide,
-                             int TgtID, int si, int ei, int sj, int ej, int sk, int ek) {
+                             int nv, int vectors, double *tgt, int along_x2, int DirInv,
+                             int TgtSide, int TgtID, int si, int ei, int sj, int ej,
+                             int sk, int ek) {
   int N = s->N;
   int n_start = -N / 2; /* local_lx = 0, meshblock_size = N */
   int nkk = ek - sk + 1, njj = ej - sj + 1, nii = ei - si + 1;
@@ -467,7 +477,7 @@ static void interpolate_edge(const or_solver *s, const or_panel *P, const double
             o1 = C4(0, sa1, j, i);
           }
           size_t vs = (size_t)s->nc3 * s->nc2 * s->nc1;
-          if (n == IVY || n == IVZ) {
+          if (vectors && (n == IVY || n == IVZ)) {
             double v1y = src[IVY * vs + o0], v1z = src[IVZ * vs + o0];
             double v2y = src[IVY * vs + o1], v2z = src[IVZ * vs + o1];
             double tgt_cy, tgt_cz, src_cy, src_cz;
@@ -577,7 +587,7 @@ static void exchange(or_solver *s, int which) {
       int sk = ox3 > 0 ? s->ke - NG + 1 : s->ks;
       int ek = ox3 < 0 ? s->ks + NG - 1 : s->ke;
       int along_x2 = (ox3 != 0);
-      interpolate_edge(s, P, src, nv, itp, along_x2, DINV[p][d], TGSIDE[p][d], TGBID[p][d],
+      interpolate_edge(s, P, src, nv, which == 0, itp, along_x2, DINV[p][d], TGSIDE[p][d], TGBID[p][d],
                        si, ei, sj, ej, sk, ek);
       pack_r(itp, nv, ek - sk + 1, ej - sj + 1, ei - si + 1, PACKR[p][d],
              bufs + (size_t)(p * 4 + d) * bsz);
