@@ -399,7 +399,10 @@ EXO_HD double div_rn(double x, double d, double y) {
 
 // ---------------------------------------------------------------------------
 // 5x5 algebra of the implicit column solve (Eigen in full_correction.cpp):
-// k-ordered sums and LU with partial pivoting, the same as the oracle.
+// k-ordered sums and LU with partial pivoting, the same as the oracle.  Sums
+// of products are fused multiply-add chains (t = fma(a_k, b_k, t) for k = 1..4
+// after t = a_0 b_0), as Eigen's fixed-size products and its LU rank-1 update
+// are when built for FMA hardware; the oracle uses the same chains.
 struct M5 {
   double a[5][5];
 };
@@ -415,7 +418,7 @@ EXO_HD M5 m5_mul(const M5 &A, const M5 &B) {
     for (int c = 0; c < 5; ++c) {
       double t = A.a[r][0] * B.a[0][c];
 #pragma unroll
-      for (int k = 1; k < 5; ++k) t = t + A.a[r][k] * B.a[k][c];
+      for (int k = 1; k < 5; ++k) t = fma(A.a[r][k], B.a[k][c], t);
       C.a[r][c] = t;
     }
   return C;
@@ -427,7 +430,7 @@ EXO_HD V5 m5_mulv(const M5 &A, const V5 &x) {
   for (int r = 0; r < 5; ++r) {
     double t = A.a[r][0] * x.a[0];
 #pragma unroll
-    for (int k = 1; k < 5; ++k) t = t + A.a[r][k] * x.a[k];
+    for (int k = 1; k < 5; ++k) t = fma(A.a[r][k], x.a[k], t);
     y.a[r] = t;
   }
   return y;
@@ -473,7 +476,7 @@ EXO_HD M5 m5_inv(const M5 &A) {
       double l = div_rn(lu[r][c], lu[c][c], yc);
       lu[r][c] = l;
 #pragma unroll
-      for (int q = c + 1; q < 5; ++q) lu[r][q] = lu[r][q] - l * lu[c][q];
+      for (int q = c + 1; q < 5; ++q) lu[r][q] = fma(-l, lu[c][q], lu[r][q]);
     }
   }
   M5 X;
@@ -484,14 +487,14 @@ EXO_HD M5 m5_inv(const M5 &A) {
     for (int r = 0; r < 5; ++r) {
       double t = piv[r] == col ? 1.0 : 0.0;
 #pragma unroll
-      for (int q = 0; q < r; ++q) t = t - lu[r][q] * y[q];
+      for (int q = 0; q < r; ++q) t = fma(-lu[r][q], y[q], t);
       y[r] = t;
     }
 #pragma unroll
     for (int r = 4; r >= 0; --r) {
       double t = y[r];
 #pragma unroll
-      for (int q = r + 1; q < 5; ++q) t = t - lu[r][q] * X.a[q][col];
+      for (int q = r + 1; q < 5; ++q) t = fma(-lu[r][q], X.a[q][col], t);
       X.a[r][col] = div_rn(t, lu[r][r], rd[r]);
     }
   }
@@ -513,30 +516,52 @@ EXO_HD void roe_average(double *prim, double gm1, const double *wl, const double
               (gm1 + 1.) * prim[IDN];
 }
 
-// R |Lambda| R^-1 from Eigenvalue / Eigenvector (flux_decomposition.hpp:63-96)
+// R |Lambda| R^-1 from Eigenvalue / Eigenvector (flux_decomposition.hpp:63-96).
+// The product runs over the structurally non-zero terms only: R (|Lambda|
+// scales its columns) has zeros at (0,3) (0,4) (1,3) (1,4) (2,4) (3,3), and
+// rows 3 and 4 of R^-1 are (-v, 0, 1, 0, 0) and (-w, 0, 0, 1, 0).  A skipped
+// term is an exact zero, so for finite entries each sum equals the dense one.
 EXO_HD M5 abs_a(const double *prim, double cs, double gm1) {
   double r = prim[IDN], u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR];
   double ke = 0.5 * (u * u + v * v + w * w);
   double hp = (gm1 + 1.) / gm1 * p / r;
   double h = hp + ke;
-  double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
-  M5 RL = {{{1. * lam[0], 1. * lam[1], 1. * lam[2], 0. * lam[3], 0. * lam[4]},
-            {(u - cs) * lam[0], u * lam[1], (u + cs) * lam[2], 0. * lam[3], 0. * lam[4]},
-            {v * lam[0], v * lam[1], v * lam[2], 1. * lam[3], 0. * lam[4]},
-            {w * lam[0], w * lam[1], w * lam[2], 0. * lam[3], 1. * lam[4]},
-            {(h - u * cs) * lam[0], ke * lam[1], (h + u * cs) * lam[2], v * lam[3],
-             w * lam[4]}}};
+  double l0 = fabs(u - cs), l1 = fabs(u), l2 = fabs(u + cs), l3 = fabs(u), l4 = fabs(u);
+  // columns 0..2 of R |Lambda|
+  double RL[5][3] = {{1. * l0, 1. * l1, 1. * l2},
+                     {(u - cs) * l0, u * l1, (u + cs) * l2},
+                     {v * l0, v * l1, v * l2},
+                     {w * l0, w * l1, w * l2},
+                     {(h - u * cs) * l0, ke * l1, (h + u * cs) * l2}};
+  // columns 3 and 4 of R |Lambda|: (R(2,3), R(4,3)) = (1, v), (R(3,4), R(4,4)) = (1, w)
+  double r23 = 1. * l3, r43 = v * l3, r34 = 1. * l4, r44 = w * l4;
   double d1 = 2. * cs * hp, d2 = 2. * hp;
   double y1 = 1. / d1, y2 = 1. / d2, y3 = 1. / hp;
-  M5 Ri = {{{div_rn(cs * ke + u * hp, d1, y1), div_rn(-hp - cs * u, d1, y1),
-             div_rn(-v, d2, y2), div_rn(-w, d2, y2), y2},
-            {div_rn(hp - ke, hp, y3), div_rn(u, hp, y3), div_rn(v, hp, y3), div_rn(w, hp, y3),
-             div_rn(-1., hp, y3)},
-            {div_rn(cs * ke - u * hp, d1, y1), div_rn(hp - cs * u, d1, y1),
-             div_rn(-v, d2, y2), div_rn(-w, d2, y2), y2},
-            {-v, 0., 1., 0., 0.},
-            {-w, 0., 0., 1., 0.}}};
-  return m5_mul(RL, Ri);
+  double Ri[3][5] = {{div_rn(cs * ke + u * hp, d1, y1), div_rn(-hp - cs * u, d1, y1),
+                      div_rn(-v, d2, y2), div_rn(-w, d2, y2), y2},
+                     {div_rn(hp - ke, hp, y3), div_rn(u, hp, y3), div_rn(v, hp, y3),
+                      div_rn(w, hp, y3), div_rn(-1., hp, y3)},
+                     {div_rn(cs * ke - u * hp, d1, y1), div_rn(hp - cs * u, d1, y1),
+                      div_rn(-v, d2, y2), div_rn(-w, d2, y2), y2}};
+  M5 A;
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      double t = RL[a][0] * Ri[0][c];
+      t = fma(RL[a][1], Ri[1][c], t);
+      A.a[a][c] = fma(RL[a][2], Ri[2][c], t);
+    }
+  // row 3 of R^-1 enters through R(.,3), row 4 through R(.,4)
+  A.a[2][0] = fma(r23, -v, A.a[2][0]);
+  A.a[2][2] = fma(r23, 1., A.a[2][2]);
+  A.a[4][0] = fma(r43, -v, A.a[4][0]);
+  A.a[4][2] = fma(r43, 1., A.a[4][2]);
+  A.a[3][0] = fma(r34, -w, A.a[3][0]);
+  A.a[3][3] = fma(r34, 1., A.a[3][3]);
+  A.a[4][0] = fma(r44, -w, A.a[4][0]);
+  A.a[4][3] = fma(r44, 1., A.a[4][3]);
+  return A;
 }
 
 // FluxJacobian, flux_decomposition.hpp:102-120 (direction x1)

@@ -640,21 +640,17 @@ struct StageW {
   int first;
 };
 
+// One cell (p, k, i, j) of the stage: the weighted average, AddFluxDivergence,
+// the coordinate source terms and the sources.  out[] receives the hydro
+// result: the tendency (ideal gas, whose state u is the weighted average and
+// is written here) or the new state (shallow water).  Passive scalars are
+// updated in place.
 template <bool MIX>
-__device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
-  size_t t = (size_t)lb * BLK + threadIdx.x;
-  int N = g.N, ni = g.nx1;
-  size_t total = (size_t)6 * N * ni * N;
-  if (t >= total) return;
-  int j = g.js + (int)(t % N);
-  size_t r = t / N;
-  int i = g.is + (int)(r % ni);
-  r /= ni;
-  int k = g.ks + (int)(r % N), p = (int)(r / N);
+__device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
+                                            int j, StageW sw, double *out) {
   double dt = st.dtv[0];
   double wght = sw.beta * dt;
   bool tend = g.eos == EOS_IDEAL;
-  double out[5];
   double a1l = 0., a1r = 0.;  // x1 faces: shallow water has no x1 flux
   if (g.eos != EOS_SW) {
     a1l = g.area1(p, k, j, i);
@@ -774,9 +770,6 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
       out[IVZ] += wght * a3c;
     }
   }
-  ADDR(1) double *dst = tend ? st.du : st.u;
-  for (int n = 0; n < g.nv; ++n)
-    if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
   // passive scalars: weighted average and flux divergence into s
   for (int n = 0; n < g.nt; ++n) {
     size_t c = g.c(n, p, k, i, j);
@@ -811,6 +804,24 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
     s -= wght * dflx / vol;
     st.s[c] = s;
   }
+}
+
+template <bool MIX>
+__device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * N * ni * N;
+  if (t >= total) return;
+  int j = g.js + (int)(t % N);
+  size_t r = t / N;
+  int i = g.is + (int)(r % ni);
+  r /= ni;
+  int k = g.ks + (int)(r % N), p = (int)(r / N);
+  double out[5];
+  update_cell<MIX>(g, st, p, k, i, j, sw, out);
+  ADDR(1) double *dst = g.eos == EOS_IDEAL ? st.du : st.u;
+  for (int n = 0; n < g.nv; ++n)
+    if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
 }
 
 template <bool MIX>
@@ -872,7 +883,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   double a23 = dx2a * dx3a;
   const double *x1f = geo, *x1v = geo + IMPL_MAXNC1 + 1, *dx1f = geo + 2 * (IMPL_MAXNC1 + 1);
   const ADDR(1) double *__restrict__ W = st.w;
-  ADDR(1) double *__restrict__ DU = st.du;
+  const ADDR(1) double *__restrict__ DU = st.du;
   ADDR(1) double *__restrict__ U = st.u;
   ADDR(1) double *__restrict__ CF = st.coef;
   size_t vs = g.vstride, jp = g.jp;
@@ -1007,12 +1018,12 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
     }
   }
   // backward substitution; delta(ie) is final.  The coefficients of level
-  // i - 1 are loaded while level i is finished.
+  // i - 1 are loaded while level i is finished.  Only u += delta is stored:
+  // nothing reads the correction itself afterwards.
   V5 dnext = dprev;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
     size_t cc = n * vs + c0 + (size_t)ie * jp;
-    DU[cc] = dnext.a[n];
     U[cc] += dnext.a[n];
   }
   M5 An;
@@ -1044,11 +1055,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
 #pragma unroll
     for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      size_t cc = n * vs + c0 + (size_t)i * jp;
-      DU[cc] = d.a[n];
-      U[cc] = uo.a[n] + d.a[n];
-    }
+    for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = uo.a[n] + d.a[n];
     dnext = d;
   }
 }
@@ -1062,335 +1069,6 @@ __global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Wo
   }
 }
 
-
-// ---------------------------------------------------------------------------
-// FullCorrection with one column per lane group: 5 lanes per column (12
-// columns per wavefront), lane r owns row r of every 5x5 block.  Matrices a
-// lane needs whole (R^-1 of the Roe state, the block being inverted, c, the
-// previous a^-1 c) pass through LDS; the LU factorisation is repeated in every
-// lane of the group and lane r back-substitutes column r of the inverse.  The
-// arithmetic per entry is the same sequence as m5_mul / m5_inv, so the result
-// is identical to k_implicit.
-constexpr int IG = 5;                 // lanes per column
-constexpr int IGPW = 12;              // columns per 64-lane block
-constexpr int ILDS = 112;             // doubles of LDS per column group
-
-__device__ __forceinline__ double sel5(int r, double a0, double a1, double a2, double a3,
-                                       double a4) {
-  return r == 0 ? a0 : (r == 1 ? a1 : (r == 2 ? a2 : (r == 3 ? a3 : a4)));
-}
-
-// row r of R |Lambda| (flux_decomposition.hpp:63-96), given the full R^-1 in L
-__device__ __forceinline__ void absa_row(int r, const double *prim, double cs, double gm1,
-                                         double *Lri, double *out, bool write_ri) {
-  double u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR], rr = prim[IDN];
-  double ke = 0.5 * (u * u + v * v + w * w);
-  double hp = (gm1 + 1.) / gm1 * p / rr;
-  double h = hp + ke;
-  double lam0 = fabs(u - cs), lam1 = fabs(u), lam2 = fabs(u + cs), lam3 = fabs(u),
-         lam4 = fabs(u);
-  if (write_ri) {
-    // R^-1 row r (rows 3, 4 are constants but written all the same)
-    double ri[5];
-    double d1 = 2. * cs * hp, d2 = 2. * hp;
-    if (r == 0) {
-      double y1 = 1. / d1, y2 = 1. / d2;
-      ri[0] = div_rn(cs * ke + u * hp, d1, y1);
-      ri[1] = div_rn(-hp - cs * u, d1, y1);
-      ri[2] = div_rn(-v, d2, y2);
-      ri[3] = div_rn(-w, d2, y2);
-      ri[4] = y2;
-    } else if (r == 1) {
-      double y3 = 1. / hp;
-      ri[0] = div_rn(hp - ke, hp, y3);
-      ri[1] = div_rn(u, hp, y3);
-      ri[2] = div_rn(v, hp, y3);
-      ri[3] = div_rn(w, hp, y3);
-      ri[4] = div_rn(-1., hp, y3);
-    } else if (r == 2) {
-      double y1 = 1. / d1, y2 = 1. / d2;
-      ri[0] = div_rn(cs * ke - u * hp, d1, y1);
-      ri[1] = div_rn(hp - cs * u, d1, y1);
-      ri[2] = div_rn(-v, d2, y2);
-      ri[3] = div_rn(-w, d2, y2);
-      ri[4] = y2;
-    } else if (r == 3) {
-      ri[0] = -v; ri[1] = 0.; ri[2] = 1.; ri[3] = 0.; ri[4] = 0.;
-    } else {
-      ri[0] = -w; ri[1] = 0.; ri[2] = 0.; ri[3] = 1.; ri[4] = 0.;
-    }
-    for (int q = 0; q < 5; ++q) Lri[r * 5 + q] = ri[q];
-    return;
-  }
-  double rl[5];
-  rl[0] = sel5(r, 1. * lam0, (u - cs) * lam0, v * lam0, w * lam0, (h - u * cs) * lam0);
-  rl[1] = sel5(r, 1. * lam1, u * lam1, v * lam1, w * lam1, ke * lam1);
-  rl[2] = sel5(r, 1. * lam2, (u + cs) * lam2, v * lam2, w * lam2, (h + u * cs) * lam2);
-  rl[3] = sel5(r, 0. * lam3, 0. * lam3, 1. * lam3, 0. * lam3, v * lam3);
-  rl[4] = sel5(r, 0. * lam4, 0. * lam4, 0. * lam4, 1. * lam4, w * lam4);
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    double t = rl[0] * Lri[0 * 5 + c];
-#pragma unroll
-    for (int k = 1; k < 5; ++k) t = t + rl[k] * Lri[k * 5 + c];
-    out[c] = t;
-  }
-}
-
-// row r of FluxJacobian (flux_decomposition.hpp:102-120)
-__device__ __forceinline__ void jac_row(int r, double gm1, const double *w, double *out) {
-  double v1 = w[IVX], v2 = w[IVY], v3 = w[IVZ], rho = w[IDN], pres = w[IPR];
-  double s2 = v1 * v1 + v2 * v2 + v3 * v3;
-  double c1 = ((gm1 - 1) * s2 / 2 - (gm1 + 1) / gm1 * pres / rho) * v1;
-  double c2 = (gm1 + 1) / gm1 * pres / rho + s2 / 2 - gm1 * v1 * v1;
-  out[0] = sel5(r, 0, gm1 * s2 / 2 - v1 * v1, -v1 * v2, -v1 * v3, c1);
-  out[1] = sel5(r, 1., (2. - gm1) * v1, v2, v3, c2);
-  out[2] = sel5(r, 0., -gm1 * v2, v1, 0., -gm1 * v2 * v1);
-  out[3] = sel5(r, 0., -gm1 * v3, 0., v1, -gm1 * v3 * v1);
-  out[4] = sel5(r, 0., gm1, 0., 0., (gm1 + 1) * v1);
-}
-
-// LU with partial pivoting of the 5x5 in T (repeated per lane) and column
-// `col` of the inverse, m5_inv's arithmetic
-__device__ __forceinline__ void inv_col(const double *T, int col, double *xcol) {
-  double lu[5][5], rd[5];
-  int piv[5];
-#pragma unroll
-  for (int r = 0; r < 5; ++r) {
-#pragma unroll
-    for (int c = 0; c < 5; ++c) lu[r][c] = T[r * 5 + c];
-    piv[r] = r;
-  }
-#pragma unroll
-  for (int c = 0; c < 5; ++c) {
-    int p = c;
-    double best = fabs(lu[c][c]);
-#pragma unroll
-    for (int r = c + 1; r < 5; ++r)
-      if (fabs(lu[r][c]) > best) {
-        best = fabs(lu[r][c]);
-        p = r;
-      }
-#pragma unroll
-    for (int r = c + 1; r < 5; ++r) {
-      if (p == r) {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          double t = lu[c][q];
-          lu[c][q] = lu[r][q];
-          lu[r][q] = t;
-        }
-        int t = piv[c];
-        piv[c] = piv[r];
-        piv[r] = t;
-      }
-    }
-    double yc = 1.0 / lu[c][c];
-    rd[c] = yc;
-#pragma unroll
-    for (int r = c + 1; r < 5; ++r) {
-      double l = div_rn(lu[r][c], lu[c][c], yc);
-      lu[r][c] = l;
-#pragma unroll
-      for (int q = c + 1; q < 5; ++q) lu[r][q] = lu[r][q] - l * lu[c][q];
-    }
-  }
-  double y[5];
-#pragma unroll
-  for (int r = 0; r < 5; ++r) {
-    double t = piv[r] == col ? 1.0 : 0.0;
-#pragma unroll
-    for (int q = 0; q < r; ++q) t = t - lu[r][q] * y[q];
-    y[r] = t;
-  }
-#pragma unroll
-  for (int r = 4; r >= 0; --r) {
-    double t = y[r];
-#pragma unroll
-    for (int q = r + 1; q < 5; ++q) t = t - lu[r][q] * xcol[q];
-    xcol[r] = div_rn(t, lu[r][r], rd[r]);
-  }
-}
-
-__device__ __forceinline__ void implicit_grp_body(const DGrid &g, const DState &st, int lb,
-                                                  double beta, double *lds, double *lap) {
-  int lane = threadIdx.x;
-  int grp = lane / IG, r = lane % IG;
-  bool lane_ok = grp < IGPW;
-  if (!lane_ok) { grp = 0; r = 0; }
-  int N = g.N;
-  int nbx = (N * N + IGPW - 1) / IGPW;
-  int col = (lb % nbx) * IGPW + grp;
-  bool act = lane_ok && col < N * N;
-  int p = lb / nbx;
-  int k = g.ks + (act ? col / N : 0), j = g.js + (act ? col % N : 0);
-  double *L = lds + grp * ILDS;
-  double *Lri = L, *LT = L + 25, *LX = L + 50, *LC = L + 75, *Ltv = L + 100, *Ldp = L + 105;
-  double *LA = lap + grp * 25;  // a^-1 c of the previous level
-  int is = g.is, ie = g.ie;
-  double dt = beta * st.dtv[0];
-  double gamma = g.gamma, grav = g.grav;
-  double cth = act ? g.T(T_COSC, p, k, j) : 0., sth = act ? g.T(T_SINC, p, k, j) : 1.;
-  auto wproj = [&](int i, double *w) {
-    for (int n = 0; n < 5; ++n) w[n] = act ? st.w[g.c(n, p, k, i, j)] : 1.;
-    w[IVY] += w[IVZ] * cth;
-    w[IVZ] *= sth;
-  };
-  double gm1c = (gamma - 1.) * 1. / 1.;
-  double gm1 = 0.5 * (gm1c + gm1c);
-  double ydt = 1. / dt;
-  double wm[5], w0[5], wp[5], prim[5];
-  wproj(is - 1, wm);
-  wproj(is, w0);
-  roe_average(prim, gm1, wm, w0);
-  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
-  double Am[5], Ap[5];
-  if (lane_ok) absa_row(r, prim, cs, gm1, Lri, nullptr, true);
-  __syncthreads();
-  if (lane_ok) absa_row(r, prim, cs, gm1, Lri, Am, false);
-  __syncthreads();
-  size_t cs30 = act ? (size_t)g.c(0, p, k, 0, j) : 0;
-  for (int i = is; i <= ie; ++i) {
-    wproj(i + 1, wp);
-    roe_average(prim, gm1, w0, wp);
-    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
-    if (lane_ok) absa_row(r, prim, cs, gm1, Lri, nullptr, true);
-    __syncthreads();
-    if (lane_ok) absa_row(r, prim, cs, gm1, Lri, Ap, false);
-    double Jm[5], J0[5], Jp[5];
-    jac_row(r, gm1c, wm, Jm);
-    jac_row(r, gm1c, w0, J0);
-    jac_row(r, gm1c, wp, Jp);
-    double aleft = act ? g.area1(p, k, j, i) : 1., aright = act ? g.area1(p, k, j, i + 1) : 1.;
-    double v = act ? g.volume(p, k, j, i) : 1.;
-    double v2 = 2. * v, yv2 = 1. / v2;
-    double a[5], b[5], c[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      double dtq = r == q ? ydt : 0.;
-      double phi = (r == IVX && q == IDN) || (r == IEN && q == IVX) ? grav : 0.;
-      a[q] = div_rn(Am[q] * aleft + Ap[q] * aright + (aright - aleft) * J0[q], v2, yv2) + dtq -
-             phi;
-      b[q] = div_rn(-(Am[q] + Jm[q]) * aleft, v2, yv2);
-      c[q] = div_rn(-(Ap[q] - Jp[q]) * aright, v2, yv2);
-    }
-    if (i == is)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) a[q] += b[q] * (q == IVX ? -1. : 1.);
-    if (i == ie)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) a[q] += c[q] * (q == IVX ? -1. : 1.);
-    double rhs = act ? div_rn(st.du[g.c(r, p, k, i, j)], dt, ydt) : 0.;
-    double T[5], tv;
-    if (i == is) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) T[q] = a[q];
-      tv = rhs;
-    } else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        double t = b[0] * LA[0 * 5 + q];
-#pragma unroll
-        for (int kk = 1; kk < 5; ++kk) t = t + b[kk] * LA[kk * 5 + q];
-        T[q] = a[q] - t;
-      }
-      double t = b[0] * Ldp[0];
-#pragma unroll
-      for (int kk = 1; kk < 5; ++kk) t = t + b[kk] * Ldp[kk];
-      tv = rhs - t;
-    }
-    if (lane_ok) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        LT[r * 5 + q] = T[q];
-        LC[r * 5 + q] = c[q];
-      }
-      Ltv[r] = tv;
-    }
-    __syncthreads();
-    double xcol[5];
-    inv_col(LT, r, xcol);
-    if (lane_ok)
-#pragma unroll
-      for (int q = 0; q < 5; ++q) LX[q * 5 + r] = xcol[q];
-    __syncthreads();
-    double xr[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) xr[q] = LX[r * 5 + q];
-    double d = xr[0] * Ltv[0];
-#pragma unroll
-    for (int kk = 1; kk < 5; ++kk) d = d + xr[kk] * Ltv[kk];
-    double An[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      double t = xr[0] * LC[0 * 5 + q];
-#pragma unroll
-      for (int kk = 1; kk < 5; ++kk) t = t + xr[kk] * LC[kk * 5 + q];
-      An[q] = t;
-    }
-    __syncthreads();  // everyone has read LA / Ldp / LX / LC of this level
-    if (lane_ok) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) LA[r * 5 + q] = An[q];
-      Ldp[r] = d;
-    }
-    if (act) {
-      ADDR(1) double *cf = st.coef;
-#pragma unroll
-      for (int q = 0; q < 5; ++q)
-        cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp] = An[q];
-      cf[(size_t)(25 + r) * g.vstride + cs30 + (size_t)i * g.jp] = d;
-    }
-#pragma unroll
-    for (int q = 0; q < 5; ++q) Am[q] = Ap[q];
-    for (int n = 0; n < 5; ++n) {
-      wm[n] = w0[n];
-      w0[n] = wp[n];
-    }
-    __syncthreads();
-  }
-  // backward substitution: delta(ie) is final; Ldp holds it
-  double dn = Ldp[r];
-  if (act) {
-    size_t cc = g.c(r, p, k, ie, j);
-    st.du[cc] = dn;
-    st.u[cc] += dn;
-  }
-  ADDR(1) double *cf = st.coef;
-  for (int i = ie - 1; i >= is; --i) {
-    double A[5], d = 0.;
-    if (act) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) A[q] = cf[(size_t)(r * 5 + q) * g.vstride + cs30 + (size_t)i * g.jp];
-      d = cf[(size_t)(25 + r) * g.vstride + cs30 + (size_t)i * g.jp];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) A[q] = 0.;
-    }
-    double t = A[0] * Ldp[0];
-#pragma unroll
-    for (int kk = 1; kk < 5; ++kk) t = t + A[kk] * Ldp[kk];
-    d = d - t;
-    __syncthreads();
-    if (lane_ok) Ldp[r] = d;
-    if (act) {
-      size_t cc = g.c(r, p, k, i, j);
-      st.du[cc] = d;
-      st.u[cc] += d;
-    }
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(64) k_implicit_grp(const Inst *__restrict__ I_, Work w, double beta) {
-  __shared__ double lds[IGPW * ILDS];
-  __shared__ double lap[IGPW * 25];
-  WORK_LOOP(w) {
-    WORK_INST(w);
-    implicit_grp_body(g, st, lb, beta, lds, lap);
-    __syncthreads();
-  }
-}
 
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
 __device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, int lb) {
@@ -2003,7 +1681,6 @@ struct Launcher {
   int nblk[K_NKIND] = {};
   int cap[K_NKIND] = {};    // workgroups of a persistent launch (0: one per block)
   size_t lds_x1 = 0;        // dynamic LDS of the x1 kernels (largest instance)
-  bool per_lane = false;    // implicit solve with one column per lane
   // live per-kernel timing with HIP events on the stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -2076,7 +1753,7 @@ hipEvent_t next_event(Launcher &L) {
 }
 
 // blocks of one instance in a launch of the given kind (0: not applicable)
-int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, bool per_lane) {
+int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
   bool ideal = g.eos == EOS_IDEAL;
   size_t cells = (size_t)6 * g.N * g.N * g.nx1;
   size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
@@ -2095,7 +1772,7 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, bool per_
     case K_UPD_MIX: return mixed ? (int)grid1(cells) : 0;
     case K_IMPL:
       if (!ideal || !g.implicit) return 0;
-      return (int)(per_lane ? grid1(cols, 64) : grid1(cols, IGPW)) * 6;
+      return (int)grid1(cols, 64) * 6;
     case K_ADD_DU: return ideal && !g.implicit ? (int)grid1(cells) : 0;
     case K_CELLS: return (int)grid1(cells);
     case K_ONE: return 1;
@@ -2126,7 +1803,7 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     long long acc = 0;
     for (int i = 0; i < n; ++i) {
       tab[(size_t)kind * (n + 1) + i] = (int)acc;
-      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned, L.per_lane);
+      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned);
     }
     if (acc > 0x7fffffffLL) throw std::string("work list too large for one launch");
     tab[(size_t)kind * (n + 1) + n] = (int)acc;
@@ -2291,11 +1968,6 @@ void init_handle(exo_handle *h) {
     st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
   }
   if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
-  {
-    // one column per lane unless EXO_IMPLICIT_GROUP=1 (5 lanes per column)
-    const char *e = getenv("EXO_IMPLICIT_GROUP");
-    h->L.per_lane = !(e && e[0] == '1');
-  }
   if (h->mixed) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
     for (int d = 0; d < 3; ++d) {
@@ -2349,8 +2021,7 @@ void launch_update(Launcher &L, int stage) {
 }
 
 void launch_implicit(Launcher &L, double beta) {
-  if (L.per_lane) launch(L, K_IMPL, KID_IMPLICIT, k_implicit, 64, 0, beta);
-  else launch(L, K_IMPL, KID_IMPLICIT, k_implicit_grp, 64, 0, beta);
+  launch(L, K_IMPL, KID_IMPLICIT, k_implicit, 64, 0, beta);
   launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
   HIPCHK(hipGetLastError());
 }
@@ -2934,7 +2605,6 @@ int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **o
   std::vector<void *> allocs;
   try {
     HIPCHK(hipStreamCreateWithFlags(&b->L.stream, hipStreamNonBlocking));
-    b->L.per_lane = members[0]->L.per_lane;
     for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(members[i]->stream));
     launcher_setup(b->L, b->members, allocs);
     void *p;

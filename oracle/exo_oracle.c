@@ -1240,7 +1240,11 @@ void or_calc_fluxes(or_solver *s) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* 5x5 algebra for the implicit solver (Eigen in the reference)              */
+/* 5x5 algebra for the implicit solver (Eigen in the reference).  Sums of    */
+/* products are fused multiply-add chains, t = fma(a_k, b_k, t) for k = 1..4 */
+/* after t = a_0 b_0, as Eigen's fixed-size products and its LU rank-1       */
+/* update are when built for FMA hardware.  fma() is exactly rounded, so the */
+/* chains give the same bits on any IEEE machine.                            */
 typedef struct { double a[5][5]; } m5;
 typedef struct { double a[5]; } v5;
 
@@ -1249,7 +1253,7 @@ static m5 m5_mul(const m5 *A, const m5 *B) {
   for (int r = 0; r < 5; ++r)
     for (int c = 0; c < 5; ++c) {
       double t = A->a[r][0] * B->a[0][c];
-      for (int k = 1; k < 5; ++k) t = t + A->a[r][k] * B->a[k][c];
+      for (int k = 1; k < 5; ++k) t = fma(A->a[r][k], B->a[k][c], t);
       C.a[r][c] = t;
     }
   return C;
@@ -1259,7 +1263,7 @@ static v5 m5_mulv(const m5 *A, const v5 *x) {
   v5 y;
   for (int r = 0; r < 5; ++r) {
     double t = A->a[r][0] * x->a[0];
-    for (int k = 1; k < 5; ++k) t = t + A->a[r][k] * x->a[k];
+    for (int k = 1; k < 5; ++k) t = fma(A->a[r][k], x->a[k], t);
     y.a[r] = t;
   }
   return y;
@@ -1283,7 +1287,7 @@ static m5 m5_inv(const m5 *A) {
     for (int r = c + 1; r < 5; ++r) {
       double l = lu[r][c] / lu[c][c];
       lu[r][c] = l;
-      for (int q = c + 1; q < 5; ++q) lu[r][q] = lu[r][q] - l * lu[c][q];
+      for (int q = c + 1; q < 5; ++q) lu[r][q] = fma(-l, lu[c][q], lu[r][q]);
     }
   }
   m5 X;
@@ -1291,12 +1295,12 @@ static m5 m5_inv(const m5 *A) {
     double y[5];
     for (int r = 0; r < 5; ++r) {
       double t = piv[r] == col ? 1.0 : 0.0;
-      for (int q = 0; q < r; ++q) t = t - lu[r][q] * y[q];
+      for (int q = 0; q < r; ++q) t = fma(-lu[r][q], y[q], t);
       y[r] = t;
     }
     for (int r = 4; r >= 0; --r) {
       double t = y[r];
-      for (int q = r + 1; q < 5; ++q) t = t - lu[r][q] * X.a[q][col];
+      for (int q = r + 1; q < 5; ++q) t = fma(-lu[r][q], X.a[q][col], t);
       X.a[r][col] = t / lu[r][r];
     }
   }
@@ -1318,6 +1322,11 @@ static void roe_average(double *prim, double gm1, const double *wl, const double
               (gm1 + 1.) * prim[IDN];
 }
 
+/* R |Lambda| R^-1 (flux_decomposition.hpp:63-96).  The product keeps the
+ * structurally non-zero terms only: R |Lambda| is zero at (0,3) (0,4) (1,3)
+ * (1,4) (2,4) (3,3), and rows 3 and 4 of R^-1 are (-v, 0, 1, 0, 0) and
+ * (-w, 0, 0, 1, 0).  A skipped term is an exact zero, so for finite entries
+ * every sum equals the dense product's (tested against it). */
 static m5 abs_a(const double *prim, double cs, double gm1) {
   double r = prim[IDN], u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR];
   double ke = 0.5 * (u * u + v * v + w * w);
@@ -1335,11 +1344,23 @@ static m5 abs_a(const double *prim, double cs, double gm1) {
              -v / (2. * hp), -w / (2. * hp), 1. / (2. * hp)},
             {-v, 0., 1., 0., 0.},
             {-w, 0., 0., 1., 0.}}};
+  /* structural non-zeros: column k = 3, 4 of R, row k of R^-1 */
+  static const int RNZ[2][5] = {{0, 0, 1, 0, 1}, {0, 0, 0, 1, 1}};
+  static const int RINZ[2][5] = {{1, 0, 1, 0, 0}, {1, 0, 0, 1, 0}};
   double lam[5] = {fabs(u - cs), fabs(u), fabs(u + cs), fabs(u), fabs(u)};
-  m5 RL;
+  m5 RL, A;
   for (int a = 0; a < 5; ++a)
     for (int b = 0; b < 5; ++b) RL.a[a][b] = R.a[a][b] * lam[b];
-  return m5_mul(&RL, &Ri);
+  for (int a = 0; a < 5; ++a)
+    for (int c = 0; c < 5; ++c) {
+      double t = RL.a[a][0] * Ri.a[0][c];
+      t = fma(RL.a[a][1], Ri.a[1][c], t);
+      t = fma(RL.a[a][2], Ri.a[2][c], t);
+      for (int k = 3; k < 5; ++k)
+        if (RNZ[k - 3][a] && RINZ[k - 3][c]) t = fma(RL.a[a][k], Ri.a[k][c], t);
+      A.a[a][c] = t;
+    }
+  return A;
 }
 
 static m5 flux_jacobian(double gm1, const double *w) {

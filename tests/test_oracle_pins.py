@@ -4,6 +4,7 @@ plausible restatement mistake (dropped term, wrong sign or index, transposed
 operand) fails it."""
 import ctypes as C
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -206,6 +207,37 @@ def test_eigensystem(orc):
     assert np.allclose(A @ A, J @ J, rtol=1e-8, atol=1e-8 * np.abs(J @ J).max())
     # Roe average: density is the geometric mean
     assert prim[0] == pytest.approx(math.sqrt(1.1 * 0.9), rel=1e-15)
+
+
+def _fma(a, b, c):
+    # exactly rounded a * b + c (Fraction arithmetic, one rounding at the end)
+    return float(Fraction(a) * Fraction(b) + Fraction(c))
+
+
+def test_abs_a_equals_dense_product(orc):
+    # |A| sums only the structurally non-zero terms of R |Lambda| R^-1; each
+    # skipped term is an exact zero, so the result must equal the dense
+    # fused-multiply-add chain over all five terms bit for bit
+    L = orc.lib()
+    gm1 = 0.4
+    rng = np.random.default_rng(17)
+    prims = [np.array([1.0, 0.0, 0.0, 0.0, 1.0e5]), np.array([0.7, -40.0, 0.0, 12.0, 6.0e4])]
+    prims += [np.array([rng.uniform(0.2, 2), *rng.normal(0, 50, 3), rng.uniform(1e4, 2e5)])
+              for _ in range(30)]
+    for prim in prims:
+        cs = math.sqrt((1 + gm1) * prim[4] / prim[0])
+        R, Ri, A = np.zeros(25), np.zeros(25), np.zeros(25)
+        L.or_eigensystem(prim, cs, gm1, R, Ri, A)
+        R, Ri, A = R.reshape(5, 5), Ri.reshape(5, 5), A.reshape(5, 5)
+        u = prim[1]
+        lam = [abs(u - cs), abs(u), abs(u + cs), abs(u), abs(u)]
+        RL = [[R[a, b] * lam[b] for b in range(5)] for a in range(5)]
+        for a in range(5):
+            for c in range(5):
+                t = RL[a][0] * Ri[0, c]
+                for k in range(1, 5):
+                    t = _fma(RL[a][k], Ri[k, c], t)
+                assert t == A[a, c], (prim, a, c)
 
 
 def test_block_thomas_vs_dense_solve(orc):
