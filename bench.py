@@ -199,6 +199,45 @@ def ensemble(ex, precision, steps, n_inst=256, keep=16):
     }, states
 
 
+def ensemble_accuracy(ex, steps=200, keep=16):
+    """Mixed against FP64 for the first `keep` ensemble instances after `steps`
+    steps, with each instance's input-rounding floor: the FP64 result's change
+    when the initial state is perturbed at FP32 rounding (2^-24), on the FP64
+    run's dt sequence.  Shallow, finely resolved instances are dynamically
+    unstable and amplify any perturbation; the bar is <= 1e-5 or <= 4x the
+    floor, per instance."""
+    rng = np.random.default_rng(SEED)
+    Ns = rng.choice(ENS_SIZES, 256)
+    h0 = rng.uniform(2000.0, 8000.0, 256)
+    rows = []
+    for q in range(keep):
+        def cfg_of(prec):
+            return ex.make_config(nx=int(Ns[q]), nx1=1, eos=ex.EOS_SHALLOW_YZ,
+                                  rsolver=ex.RS_ROE_SW, coriolis=ex.COR_FORCING2, cfl=0.4,
+                                  x1min=R_EARTH, x1max=R_EARTH + 1.0, omega=7.292e-5,
+                                  precision=prec)
+        u0, _ = ex.init_state(cfg_of(ex.PREC_FP64), 0, SEED + q, (float(h0[q]), 50.0, 9.81))
+
+        def run(prec, u, dts=None):
+            g = ex.Solver(cfg_of(prec))
+            g.upload(u)
+            seq = [g.step(1, dts[s] if dts else 0.0) for s in range(steps)]
+            out = g.download()[0]
+            g.close()
+            return out, seq
+        ref, dts = run(ex.PREC_FP64, u0)
+        mix, _ = run(ex.PREC_MIXED, u0)
+        up = u0 * (1.0 + 2.0 ** -24 * np.random.default_rng(5).standard_normal(u0.shape))
+        per, _ = run(ex.PREC_FP64, up, dts)
+        e = float(rel_l2(mix, ref)[[0, 2, 3]].max())
+        f = float(rel_l2(per, ref)[[0, 2, 3]].max())
+        rows.append({"N": int(Ns[q]), "h0": round(float(h0[q]), 1), "rel_l2": e, "floor": f})
+    ok = sum(1 for r in rows if r["rel_l2"] <= 1e-5 or r["rel_l2"] <= 4 * r["floor"])
+    return {"steps": steps, "instances": keep, "within_bar": ok,
+            "rel_l2_max": max(r["rel_l2"] for r in rows),
+            "rel_l2_median": float(np.median([r["rel_l2"] for r in rows])), "per_instance": rows}
+
+
 def large(ex, steps):
     """6 x 384 x 384 x 128 rotating atmosphere, 2 tracers, HLLC, mixed form."""
     cfg = ex.make_config(nx=384, nx1=128, eos=ex.EOS_IDEAL, rsolver=ex.RS_HLLC,
@@ -336,6 +375,7 @@ def main():
         l2e = max(float(rel_l2(a, b)[[0, 2, 3]].max()) for a, b in zip(smx, s64))
         ens = {"fp64": e64, "mixed": emx, "mixed_over_fp64": emx["value"] / e64["value"],
                "mixed_rel_l2_vs_fp64_max": l2e,
+               "accuracy_200_steps": ensemble_accuracy(ex),
                "workload": "256 shallow-water instances, N in {24..192}, 1 layer, WENO5+Roe, "
                            "Coriolis, SSPRK3 CFL 0.4"}
         v64, r64s = serving(ex, ex.PREC_FP64, args.requests)

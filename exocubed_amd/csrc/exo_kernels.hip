@@ -530,6 +530,63 @@ __global__ void __launch_bounds__(BLK) k_edge_states(const Inst *__restrict__ I_
 // x2 faces (dir = 1) or x3 faces (dir = 2): WENO5 L/R of the face, the panel
 // edge's ghost side from the exchanged states, and Hydro::RiemannSolver with
 // the cubed-sphere frame maps.  One thread per (panel, k, i, j) face.
+// Hydro flux of the x2 (dir 1) or x3 (dir 2) face (p, k, i, j), j = js + fa
+// (x2) or k = ks + fa (x3), along-edge index kk: WENO5 L/R states, the
+// neighbour panel's state on a panel edge (fa = 0 or N), then riemann().
+template <int dir>
+__device__ __forceinline__ void face_flux64(const DGrid &g, const DState &st, int p, int kk, int fa,
+                                            int i, int k, int j, double *fl) {
+  int N = g.N, ni = g.nx1;
+  double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    if (g.skipv(n)) continue;
+    double m3, m2, m1, c0, p1, p2;
+    if (dir == 1) {
+      m3 = st.w[g.c(n, p, k, i, j - 3)];
+      m2 = st.w[g.c(n, p, k, i, j - 2)];
+      m1 = st.w[g.c(n, p, k, i, j - 1)];
+      c0 = st.w[g.c(n, p, k, i, j)];
+      p1 = st.w[g.c(n, p, k, i, j + 1)];
+      p2 = st.w[g.c(n, p, k, i, j + 2)];
+    } else {
+      m3 = st.w[g.c(n, p, k - 3, i, j)];
+      m2 = st.w[g.c(n, p, k - 2, i, j)];
+      m1 = st.w[g.c(n, p, k - 1, i, j)];
+      c0 = st.w[g.c(n, p, k, i, j)];
+      p1 = st.w[g.c(n, p, k + 1, i, j)];
+      p2 = st.w[g.c(n, p, k + 2, i, j)];
+    }
+    // on a panel edge the ghost side is the neighbour's state (below)
+    if (fa != 0) wl[n] = weno5(p1, c0, m1, m2, m3);
+    if (fa != N) wr[n] = weno5(m2, m1, c0, p1, p2);
+  }
+  size_t vs = (size_t)6 * 4 * N * ni;
+  int lo_side = dir == 1 ? 0 : 2;
+  if (fa == 0) {  // ghost side is the L state
+    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (!g.skipv(n)) wl[n] = st.edge[n * vs + b];
+  } else if (fa == N) {  // ghost side is the R state
+    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (!g.skipv(n)) wr[n] = st.edge[n * vs + b];
+  }
+  double cosf, sinf, cth;
+  if (dir == 1) {
+    cosf = g.T(T_COS2, p, k, j);
+    sinf = g.T(T_SIN2, p, k, j);
+    cth = g.T(T_CTH2, p, k, j);
+  } else {
+    cosf = g.T(T_COS3, p, k, j);
+    sinf = g.T(T_SIN3, p, k, j);
+    cth = g.T(T_CTH3, p, k, j);
+  }
+  riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
+}
+
 template <int dir>
 __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, int lb) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
@@ -558,53 +615,8 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
   int k, j;
   if (dir == 1) { k = g.ks + kk; j = g.js + fa; }
   else { k = g.ks + fa; j = g.js + kk; }
-  double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
-#pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    if (g.skipv(n)) continue;
-    double m3, m2, m1, c0, p1, p2;
-    if (dir == 1) {
-      m3 = st.w[g.c(n, p, k, i, j - 3)];
-      m2 = st.w[g.c(n, p, k, i, j - 2)];
-      m1 = st.w[g.c(n, p, k, i, j - 1)];
-      c0 = st.w[g.c(n, p, k, i, j)];
-      p1 = st.w[g.c(n, p, k, i, j + 1)];
-      p2 = st.w[g.c(n, p, k, i, j + 2)];
-    } else {
-      m3 = st.w[g.c(n, p, k - 3, i, j)];
-      m2 = st.w[g.c(n, p, k - 2, i, j)];
-      m1 = st.w[g.c(n, p, k - 1, i, j)];
-      c0 = st.w[g.c(n, p, k, i, j)];
-      p1 = st.w[g.c(n, p, k + 1, i, j)];
-      p2 = st.w[g.c(n, p, k + 2, i, j)];
-    }
-    wl[n] = weno5(p1, c0, m1, m2, m3);
-    wr[n] = weno5(m2, m1, c0, p1, p2);
-  }
-  size_t vs = (size_t)6 * 4 * N * ni;
-  int lo_side = dir == 1 ? 0 : 2;
-  if (fa == 0) {  // ghost side is the L state
-    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
-#pragma unroll
-    for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wl[n] = st.edge[n * vs + b];
-  } else if (fa == N) {  // ghost side is the R state
-    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
-#pragma unroll
-    for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wr[n] = st.edge[n * vs + b];
-  }
-  double cosf, sinf, cth;
-  if (dir == 1) {
-    cosf = g.T(T_COS2, p, k, j);
-    sinf = g.T(T_SIN2, p, k, j);
-    cth = g.T(T_CTH2, p, k, j);
-  } else {
-    cosf = g.T(T_COS3, p, k, j);
-    sinf = g.T(T_SIN3, p, k, j);
-    cth = g.T(T_CTH3, p, k, j);
-  }
-  riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
+  double fl[5];
+  face_flux64<dir>(g, st, p, kk, fa, i, k, j, fl);
   ADDR(1) double *F = st.flx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
@@ -625,6 +637,39 @@ __global__ void __launch_bounds__(BLK) k_flux_h(const Inst *__restrict__ I_, Wor
   WORK_LOOP(w) {
     WORK_INST(w);
     flux_h_body<dir>(g, st, lb);
+  }
+}
+
+// Mixed shallow water: the hydro fluxes of the faces on panel edges in FP64
+// (face_flux64), so the two panels that share an edge face apply the same
+// Riemann flux there, as in the FP64 form.  Threads cover [p][side][kk][i]:
+// side 0/1 the x2 faces fa = 0 / N, side 2/3 the x3 faces.
+__device__ __forceinline__ void edge_flux_body(const DGrid &g, const DState &st, int lb) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * 4 * N * ni;
+  if (t >= total) return;
+  int i = g.is + (int)(t % ni);
+  size_t r = t / ni;
+  int kk = (int)(r % N);
+  r /= N;
+  int side = (int)(r % 4), p = (int)(r / 4);
+  int fa = (side & 1) ? N : 0;
+  double fl[5];
+  int dir = side < 2 ? 1 : 2;
+  int k = dir == 1 ? g.ks + kk : g.ks + fa, j = dir == 1 ? g.js + fa : g.js + kk;
+  if (dir == 1) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
+  else face_flux64<2>(g, st, p, kk, fa, i, k, j, fl);
+  ADDR(1) double *F = st.flx[dir];
+#pragma unroll
+  for (int n = 0; n < 5; ++n)
+    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
+}
+
+__global__ void __launch_bounds__(BLK) k_edge_flux(const Inst *__restrict__ I_, Work w) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    edge_flux_body(g, st, lb);
   }
 }
 
@@ -689,15 +734,15 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       dflx += a2r * st.flx[1][g.f(n, p, k, i, j + 1)] - a2l * st.flx[1][g.f(n, p, k, i, j)];
       dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
     } else {
-      // FP32 perturbation fluxes plus the FP64 reference-pressure parts
+      // FP32 perturbation fluxes plus the FP64 reference-pressure parts (the
+      // ideal gas; mixed shallow water stores FP64 face fluxes and takes the
+      // FP64 update)
       const ADDR(1) float *F0 = st.fflx[0], *F1 = st.fflx[1], *F2 = st.fflx[2];
-      if (g.eos == EOS_SW) dflx = 0.;
-      else
-        dflx = a1r * (double)F0[g.f(n, p, k, i + 1, j)] - a1l * (double)F0[g.f(n, p, k, i, j)];
+      dflx = a1r * (double)F0[g.f(n, p, k, i + 1, j)] - a1l * (double)F0[g.f(n, p, k, i, j)];
       dflx += a2r * (double)F1[g.f(n, p, k, i, j + 1)] - a2l * (double)F1[g.f(n, p, k, i, j)];
       dflx += a3r * (double)F2[g.f(n, p, k + 1, i, j)] - a3l * (double)F2[g.f(n, p, k, i, j)];
       double P = st.lref[i];
-      if (n == IVX && g.eos == EOS_IDEAL)
+      if (n == IVX)
         dflx += a1r * st.psf[g.f(0, p, k, i + 1, j)] - a1l * st.psf[g.f(0, p, k, i, j)];
       if (n == IVY)
         dflx += ((a2r * g.T(T_PRY2, p, k, j + 1) - a2l * g.T(T_PRY2, p, k, j)) +
@@ -1519,6 +1564,32 @@ __global__ void __launch_bounds__(BLK) k_edge_states_mix(const Inst *__restrict_
   }
 }
 
+// Shallow-water base flux of the mixed form: F(b) = (h u, h u^2 + h^2/2, h u v)
+// for the primitive state b = (h, v2, v3) of a face's left cell, through the
+// face's frame maps and covariant projection as riemann() applies them; with
+// s = sinf and P = h^2/2 the maps reduce to
+//   x2 (dir 1): (h s v2, h s v2^2 + P/s, h s v2 v3 - cosf P/s)
+//   x3 (dir 2): (h s v3, h s v2 v3 - cosf P/s, h s v3^2 + P/s).
+// f receives (IDN, -, IVY, IVZ).
+template <int dir>
+__device__ __forceinline__ void sw_base_flux(double h, double v2, double v3, double cf, double sf,
+                                             double ct, double *f) {
+  double hs = h * sf, Ps = 0.5 * h * h / sf;
+  double fy, fz;
+  if (dir == 1) {
+    f[IDN] = hs * v2;
+    fy = hs * v2 * v2 + Ps;
+    fz = hs * v2 * v3 - cf * Ps;
+  } else {
+    f[IDN] = hs * v3;
+    fy = hs * v2 * v3 - cf * Ps;
+    fz = hs * v3 * v3 + Ps;
+  }
+  f[IVX] = 0.;
+  f[IVY] = fy + fz * ct;
+  f[IVZ] = fz + fy * ct;
+}
+
 template <int dir>
 __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st, int lb) {
 #pragma clang fp contract(fast)
@@ -1546,51 +1617,87 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   int k = dir == 1 ? g.ks + kk : g.ks + fa;
   int j = dir == 1 ? g.js + fa : g.js + kk;
   float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
+  // shallow water: deltas about the face's left cell (riemann_sw_delta)
+  bool sw = g.eos == EOS_SW;
+  double B[5] = {0., 0., 0., 0., 0.};
+  if (sw) {
+    int kL = dir == 1 ? k : k - 1, jL = dir == 1 ? j - 1 : j;
+    B[IDN] = st.w[g.c(IDN, p, kL, i, jL)];
+    B[IVY] = st.w[g.c(IVY, p, kL, i, jL)];
+    B[IVZ] = st.w[g.c(IVZ, p, kL, i, jL)];
+  }
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
     if (g.skipv(n)) continue;
     float v[6];
 #pragma unroll
-    for (int s = 0; s < 6; ++s)
-      v[s] = dir == 1 ? pert(g, st, n, p, k, i, j - 3 + s) : pert(g, st, n, p, k - 3 + s, i, j);
+    for (int s = 0; s < 6; ++s) {
+      int ks = dir == 1 ? k : k - 3 + s, js = dir == 1 ? j - 3 + s : j;
+      v[s] = sw ? (float)(st.w[g.c(n, p, ks, i, js)] - B[n]) : pert(g, st, n, p, ks, i, js);
+    }
     wl[n] = weno5f(v[4], v[3], v[2], v[1], v[0]);
     wr[n] = weno5f(v[1], v[2], v[3], v[4], v[5]);
   }
   size_t vs = (size_t)6 * 4 * N * ni;
   int lo_side = dir == 1 ? 0 : 2;
+  // panel edges: the neighbour's state (FP64 for shallow water, taken about B)
   if (fa == 0) {
     size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wl[n] = st.fedge[n * vs + b];
+      if (!g.skipv(n)) wl[n] = sw ? (float)(st.edge[n * vs + b] - B[n]) : st.fedge[n * vs + b];
   } else if (fa == N) {
     size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wr[n] = st.fedge[n * vs + b];
+      if (!g.skipv(n)) wr[n] = sw ? (float)(st.edge[n * vs + b] - B[n]) : st.fedge[n * vs + b];
   }
-  // full density for the ideal gas; shallow water keeps h' (the solver adds H)
-  float P;
-  if (g.eos == EOS_SW) {
-    P = (float)st.lref[g.nc1 + i];
-  } else {
+  // full density for the ideal gas
+  float P = 0.f;
+  if (!sw) {
     float rref = (float)st.lref[2 * g.nc1 + i];
     wl[IDN] += rref;
     wr[IDN] += rref;
     P = (float)st.lref[g.nc1 + i];
   }
-  float cosf, sinf, cth;
+  double cosd, sind, cthd;
   if (dir == 1) {
-    cosf = (float)g.T(T_COS2, p, k, j);
-    sinf = (float)g.T(T_SIN2, p, k, j);
-    cth = (float)g.T(T_CTH2, p, k, j);
+    cosd = g.T(T_COS2, p, k, j);
+    sind = g.T(T_SIN2, p, k, j);
+    cthd = g.T(T_CTH2, p, k, j);
   } else {
-    cosf = (float)g.T(T_COS3, p, k, j);
-    sinf = (float)g.T(T_SIN3, p, k, j);
-    cth = (float)g.T(T_CTH3, p, k, j);
+    cosd = g.T(T_COS3, p, k, j);
+    sind = g.T(T_SIN3, p, k, j);
+    cthd = g.T(T_CTH3, p, k, j);
   }
-  riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf,
-               sinf, 0.f);
+  float cosf = (float)cosd, sinf = (float)sind, cth = (float)cthd;
+  if (sw) {
+    // FP64 face flux = base flux + FP32 delta; the panel-edge faces are
+    // written by k_edge_flux in FP64
+    float bf[5] = {(float)B[IDN], 0.f, (float)B[IVY], (float)B[IVZ], 0.f}, mb;
+    riemann_sw_delta(g.rs, dir == 1 ? IVY : IVZ, bf, wl, wr, fl, cth, cosf, sinf, &mb);
+    double fb[4];
+    sw_base_flux<dir>(B[IDN], B[IVY], B[IVZ], cosd, sind, cthd, fb);
+    ADDR(1) double *F = st.flx[dir];
+    if (fa != 0 && fa != N) {
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fb[n] + (double)fl[n];
+    }
+    double m = fb[IDN] + (double)fl[IDN];
+    for (int n = 0; n < g.nt; ++n) {
+      float r[6];
+      for (int s = 0; s < 6; ++s)
+        r[s] = (float)(dir == 1 ? st.r[g.c(n, p, k, i, j - 3 + s)] : st.r[g.c(n, p, k - 3 + s, i, j)]);
+      float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
+      float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
+      st.sflx[dir][g.f(n, p, k, i, j)] = m >= 0.0 ? m * (double)rl : m * (double)rr;
+    }
+    return;
+  }
+  riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf, sinf,
+               0.f);
+  float m = fl[IDN];
   ADDR(1) float *F = st.fflx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
@@ -1601,7 +1708,6 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
       r[s] = (float)(dir == 1 ? st.r[g.c(n, p, k, i, j - 3 + s)] : st.r[g.c(n, p, k - 3 + s, i, j)]);
     float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
     float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
-    float m = fl[IDN];
     st.fsflx[dir][g.f(n, p, k, i, j)] = m >= 0.f ? m * rl : m * rr;
   }
 }
@@ -1660,7 +1766,7 @@ namespace {
 // launch kinds, each with its own block-prefix table over the instances
 enum {
   K_PRIM = 0, K_GHOST_U, K_GHOST_S, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX,
-  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_NKIND
+  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -1764,12 +1870,15 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     case K_GHOST_S: return g.nt ? (int)grid1((size_t)ngd * g.nx1) : 0;
     case K_X1: return ideal && !mixed ? (int)grid1(cols, 64) * 6 : 0;
     case K_X1_MIX: return ideal && mixed ? (int)grid1(cols, 64) * 6 : 0;
-    case K_EDGE: return !mixed ? (int)grid1((size_t)ned * g.nx1) : 0;
-    case K_EDGE_MIX: return mixed ? (int)grid1((size_t)ned * g.nx1) : 0;
+    // mixed shallow water takes FP64 edge states (its faces are taken about an
+    // FP64 base state)
+    case K_EDGE: return !mixed || !ideal ? (int)grid1((size_t)ned * g.nx1) : 0;
+    case K_EDGE_MIX: return mixed && ideal ? (int)grid1((size_t)ned * g.nx1) : 0;
     case K_FACE: return !mixed ? (int)grid1(faces) : 0;
     case K_FACE_MIX: return mixed ? (int)grid1(faces) : 0;
-    case K_UPD: return !mixed ? (int)grid1(cells) : 0;
-    case K_UPD_MIX: return mixed ? (int)grid1(cells) : 0;
+    // mixed shallow water stores FP64 face fluxes: the FP64 update
+    case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
+    case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
     case K_IMPL:
       if (!ideal || !g.implicit) return 0;
       return (int)grid1(cols, 64) * 6;
@@ -1777,6 +1886,7 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     case K_CELLS: return (int)grid1(cells);
     case K_ONE: return 1;
     case K_DT: return (int)grid1(cells, BLK * DTC);
+    case K_EFLX: return mixed && !ideal ? (int)grid1((size_t)6 * 4 * g.N * g.nx1) : 0;
   }
   return 0;
 }
@@ -1895,22 +2005,24 @@ size_t footprint_bytes(const Setup &s) {
   for (int q = 0; q < 4; ++q) add(cell * g.nv);           // u, w, u1, du
   if (g.nt)
     for (int q = 0; q < 3; ++q) add(cell * g.nt);         // s, r, s1
-  if (!mixed) {
+  // FP32 face fluxes and edge states for the mixed ideal gas only
+  bool f32 = mixed && g.eos == EXO_EOS_IDEAL;
+  if (!f32) {
     for (int d = 0; d < 3; ++d) add(face * g.nv);
     if (g.nt)
       for (int d = 0; d < 3; ++d) add(face * g.nt);
     add(edge * g.nv * sizeof(double));
   }
   if (g.eos == EXO_EOS_IDEAL && g.implicit) add(cell * 30);
-  if (mixed) {
+  if (f32) {
     for (int d = 0; d < 3; ++d) {
       add(g.fvstride * sizeof(float) * g.nv);
       if (g.nt) add(g.fvstride * sizeof(float) * g.nt);
     }
     add(edge * g.nv * sizeof(float));
-    if (g.eos == EXO_EOS_IDEAL) add(face);
-    add(3 * (size_t)g.nc1 * sizeof(double));
+    add(face);
   }
+  if (mixed) add(3 * (size_t)g.nc1 * sizeof(double));
   add(sizeof(unsigned long long));
   add(2 * sizeof(double));
   add((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
@@ -1961,23 +2073,27 @@ void init_handle(exo_handle *h) {
     st.s1 = (double *)h->dalloc(h->cells_bytes(g.nt));
   }
   h->mixed = s.cfg.precision == EXO_PREC_MIXED;
-  if (!h->mixed) {
+  // FP32 face fluxes and edge states for the mixed ideal gas only: mixed
+  // shallow water forms FP64 face fluxes from FP64 base fluxes and FP32
+  // deltas (flux_h_mix_body), the rest of its stage is the FP64 one
+  bool f32 = h->mixed && g.eos == EXO_EOS_IDEAL;
+  if (!f32) {
     for (int d = 0; d < 3; ++d) st.flx[d] = (double *)h->dalloc(h->face_bytes(g.nv));
     if (g.nt)
       for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
     st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
   }
   if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
-  if (h->mixed) {
+  if (f32) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
     for (int d = 0; d < 3; ++d) {
       st.fflx[d] = (float *)h->dalloc(fb * g.nv);
       if (g.nt) st.fsflx[d] = (float *)h->dalloc(fb * g.nt);
     }
     st.fedge = (float *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(float));
-    if (g.eos == EXO_EOS_IDEAL) st.psf = (double *)h->dalloc(g.fvstride * sizeof(double));
-    st.lref = (double *)h->dalloc((size_t)3 * g.nc1 * sizeof(double));
+    st.psf = (double *)h->dalloc(g.fvstride * sizeof(double));
   }
+  if (h->mixed) st.lref = (double *)h->dalloc((size_t)3 * g.nc1 * sizeof(double));
   st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
   st.dtv = (double *)h->dalloc(2 * sizeof(double));
   st.part = (double *)h->dalloc((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
@@ -2000,6 +2116,7 @@ void launch_fluxes(Launcher &L) {
   launch(L, K_X1_MIX, KID_FLUX_X1, k_flux_x1_mix, 64 * X1W, L.lds_x1);
   launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
   launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
+  launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
   launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1>, BLK, 0);

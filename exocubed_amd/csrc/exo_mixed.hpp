@@ -10,7 +10,8 @@
 // files they cite) with the pressure split out exactly:
 //   LMARS:  pbar = P + pbar',  pbar' = (p'_l + p'_r)/2 + rho c (u_l - u_r)/2
 //   HLLC:   t_l = P + t_l', cp = P + cp', sl + sr + sm = 1
-//   Roe/LMARS shallow water: (h^2)/2 about h_ref.
+// Shallow water instead takes each face about its left cell's FP64 state
+// (roe_sw_delta, lmars_sw_delta below).
 #pragma once
 
 #include "exo_device.hpp"
@@ -132,29 +133,46 @@ EXO_HD void hllc_pert(int ivx, float gamma, const float *wli, const float *wri, 
   fl[IEN] = sl * fLe + sr * fRe + sm * (P + cp) * am;
 }
 
-// shallow water: q[IDN] holds h' about href, H = href
-EXO_HD void roe_sw_pert(int ivx, const float *wl, const float *wr, float H, float *fl) {
+// Shallow water about a base state b, the FP64 primitive state (h, -, v2, v3)
+// of the face's left cell: the face states are b + dl and b + dr, with the
+// deltas reconstructed from v - b (WENO5 is invariant under a constant shift),
+// so in FP32 they carry only the local variation of the field.  The solvers
+// return F(b + dl, b + dr) - F(b), every product involving b taken against a
+// delta; F(b) = (h u, h u^2 + h^2/2, h u v) in the face frame is added back in
+// FP64 by the update.  The large geopotential flux h^2/2 and its linear part
+// h_b dh therefore never pass through FP32 as a whole.
+EXO_HD void roe_sw_delta(int ivx, const float *b, const float *dl, const float *dr, float *fl) {
   int ivy = ivx == IVY ? IVZ : IVY;
-  float hl = H + wl[IDN], hr = H + wr[IDN];
+  float hB = b[IDN], uB = b[ivx], vB = b[ivy];
+  float hl = hB + dl[IDN], hr = hB + dr[IDN];
   float sl = sqrtf(hl), sr = sqrtf(hr);
   float isd = frcp(sl + sr);
-  float ubar = (wl[ivx] * sl + wr[ivx] * sr) * isd;
-  float vbar = (wl[ivy] * sl + wr[ivy] * sr) * isd;
+  float ubar = ((uB + dl[ivx]) * sl + (uB + dr[ivx]) * sr) * isd;
+  float vbar = ((vB + dl[ivy]) * sl + (vB + dr[ivy]) * sr) * isd;
   float cbar = sqrtf(0.5f * (hl + hr));
-  float delh = wr[IDN] - wl[IDN];
-  float delu = wr[ivx] - wl[ivx];
-  float delv = wr[ivy] - wl[ivy];
+  float delh = dr[IDN] - dl[IDN];
+  float delu = dr[ivx] - dl[ivx];
+  float delv = dr[ivy] - dl[ivy];
   float hbar = sqrtf(hl * hr);
   float ic = frcp(cbar);
   float a1 = 0.5f * (cbar * delh - hbar * delu) * ic;
   float a2 = hbar * delv;
   float a3 = 0.5f * (cbar * delh + hbar * delu) * ic;
   float s0 = fabsf(ubar - cbar), s1 = fabsf(ubar), s2 = fabsf(ubar + cbar);
-  float fd = 0.5f * (hl * wl[ivx] + hr * wr[ivx]);
-  // 0.25 (hl^2 + hr^2) - H^2 / 2 = H (h'l + h'r) / 2 + (h'l^2 + h'r^2) / 4
-  float fx = 0.5f * (hl * wl[ivx] * wl[ivx] + hr * wr[ivx] * wr[ivx]) +
-             0.5f * H * (wl[IDN] + wr[IDN]) + 0.25f * (wl[IDN] * wl[IDN] + wr[IDN] * wr[IDN]);
-  float fy = 0.5f * (hl * wl[ivx] * wl[ivy] + hr * wr[ivx] * wr[ivy]);
+  // central part, 0.5 (F(b + dl) + F(b + dr)) - F(b)
+  float fd = 0.f, fx = 0.f, fy = 0.f;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const float *d = side ? dr : dl;
+    float dh = d[IDN], du = d[ivx], dv = d[ivy];
+    float u2 = 2.f * uB + du;
+    fd += hB * du + dh * (uB + du);
+    fx += dh * uB * uB + hB * du * u2 + dh * du * u2 + hB * dh + 0.5f * dh * dh;
+    fy += hB * (uB * dv + du * vB + du * dv) + dh * (uB + du) * (vB + dv);
+  }
+  fd *= 0.5f;
+  fx *= 0.5f;
+  fy *= 0.5f;
   fd -= 0.5f * (s0 * a1 + s2 * a3);
   fx -= 0.5f * (s0 * a1 * (ubar - cbar) + s2 * a3 * (ubar + cbar));
   fy -= 0.5f * (s0 * a1 * vbar + s1 * a2 + s2 * a3 * vbar);
@@ -164,22 +182,42 @@ EXO_HD void roe_sw_pert(int ivx, const float *wl, const float *wr, float H, floa
   fl[ivy] = fy;
 }
 
-EXO_HD void lmars_sw_pert(int ivx, const float *wl, const float *wr, float H, float *fl) {
+EXO_HD void lmars_sw_delta(int ivx, const float *b, const float *dl, const float *dr, float *fl) {
   int ivy = ivx == IVY ? IVZ : IVY;
-  float hl = H + wl[IDN], hr = H + wr[IDN];
-  float cbar = sqrtf(0.5f * (hl + hr));
-  float hb = 0.5f * (wl[IDN] + wr[IDN]) + 0.5f * cbar * (wl[ivx] - wr[ivx]);  // hbar - H
-  float ubar = 0.5f * (wl[ivx] + wr[ivx]) + 0.5f * frcp(cbar) * (wl[IDN] - wr[IDN]);
-  const float *w = ubar > 0.f ? wl : wr;
-  float h = ubar > 0.f ? hl : hr;
-  fl[IDN] = ubar * h;
-  fl[ivx] = ubar * h * w[ivx] + H * hb + 0.5f * hb * hb;
-  fl[ivy] = ubar * h * w[ivy];
+  float hB = b[IDN], uB = b[ivx], vB = b[ivy];
+  float cbar = sqrtf(0.5f * ((hB + dl[IDN]) + (hB + dr[IDN])));
+  float dhb = 0.5f * (dl[IDN] + dr[IDN]) + 0.5f * cbar * (dl[ivx] - dr[ivx]);          // hbar - hB
+  float dub = 0.5f * (dl[ivx] + dr[ivx]) + 0.5f * frcp(cbar) * (dl[IDN] - dr[IDN]);  // ubar - uB
+  const float *d = uB + dub > 0.f ? dl : dr;
+  float dh = d[IDN], du = d[ivx], dv = d[ivy];
+  float dm = uB * dh + dub * (hB + dh);  // ubar h - uB hB
+  fl[IDN] = dm;
+  fl[ivx] = dm * (uB + du) + uB * hB * du + hB * dhb + 0.5f * dhb * dhb;
+  fl[ivy] = dm * (vB + dv) + uB * hB * dv;
   fl[IVX] = 0.f;
 }
 
-// Riemann flux without the reference-pressure part, with the cubed-sphere
-// frame maps of the FP64 path (PrimToLocal, FluxToGlobal, covariant factor).
+// Shallow-water face flux less the base flux, with the frame maps of the FP64
+// path (the maps are linear, so the deltas map as the states do).  b, dl, dr
+// are modified (face frame); mbase receives the base mass flux h_b u_b.
+EXO_HD void riemann_sw_delta(int rs, int ivx, float *b, float *dl, float *dr, float *f, float cth,
+                             float cosf, float sinf, float *mbase) {
+  f[0] = f[1] = f[2] = f[3] = 0.f;
+  local_frame<float>(ivx, cosf, sinf, b);
+  local_frame<float>(ivx, cosf, sinf, dl);
+  local_frame<float>(ivx, cosf, sinf, dr);
+  if (rs == RS_ROE_SW) roe_sw_delta(ivx, b, dl, dr, f);
+  else lmars_sw_delta(ivx, b, dl, dr, f);
+  *mbase = b[IDN] * b[ivx];
+  flux_to_global<float>(ivx, cosf, sinf, f);
+  float ty = f[IVY], tz = f[IVZ];
+  f[IVY] = ty + tz * cth;
+  f[IVZ] = tz + ty * cth;
+}
+
+// Ideal-gas Riemann flux without the reference-pressure part, with the
+// cubed-sphere frame maps of the FP64 path (PrimToLocal, FluxToGlobal,
+// covariant factor).
 EXO_HD void riemann_pert(int eos, int rs, int ivx, float gamma, float *wl, float *wr, float P,
                          float *f, float cth, float cosf, float sinf, float cosc) {
   f[0] = f[1] = f[2] = f[3] = 0.f;
@@ -188,9 +226,7 @@ EXO_HD void riemann_pert(int eos, int rs, int ivx, float gamma, float *wl, float
     local_frame<float>(ivx, cosf, sinf, wl);
     local_frame<float>(ivx, cosf, sinf, wr);
   }
-  if (rs == RS_ROE_SW) roe_sw_pert(ivx, wl, wr, P, f);
-  else if (rs == RS_LMARS_SW) lmars_sw_pert(ivx, wl, wr, P, f);
-  else if (rs == RS_HLLC) hllc_pert(ivx, gamma, wl, wr, P, f, cosc);
+  if (rs == RS_HLLC) hllc_pert(ivx, gamma, wl, wr, P, f, cosc);
   else lmars_pert(ivx, gamma, wl, wr, P, f);
   if (rs != RS_LMARS) {
     if (ivx != IVX) flux_to_global<float>(ivx, cosf, sinf, f);
