@@ -1062,8 +1062,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
       w0[n] = wp[n];
     }
   }
-  // backward substitution; delta(ie) is final.  The coefficients of level
-  // i - 1 are loaded while level i is finished.  Only u += delta is stored:
+  // backward substitution; delta(ie) is final.  Only u += delta is stored:
   // nothing reads the correction itself afterwards.
   V5 dnext = dprev;
 #pragma unroll
@@ -1073,7 +1072,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   }
   M5 An;
   V5 dn, un;
-  if (ie - 1 >= is) {
+  if (ie - 1 >= is) {  // the coefficients and state of level ie - 1, one level ahead
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
@@ -1590,7 +1589,7 @@ __device__ __forceinline__ void sw_base_flux(double h, double v2, double v3, dou
   f[IVZ] = fz + fy * ct;
 }
 
-template <int dir>
+template <int dir, bool SW>
 __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st, int lb) {
 #pragma clang fp contract(fast)
   size_t t = (size_t)lb * BLK + threadIdx.x;
@@ -1618,7 +1617,7 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   int j = dir == 1 ? g.js + fa : g.js + kk;
   float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
   // shallow water: deltas about the face's left cell (riemann_sw_delta)
-  bool sw = g.eos == EOS_SW;
+  constexpr bool sw = SW;
   double B[5] = {0., 0., 0., 0., 0.};
   if (sw) {
     int kL = dir == 1 ? k : k - 1, jL = dir == 1 ? j - 1 : j;
@@ -1712,11 +1711,13 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   }
 }
 
-template <int dir>
+// SW: the shallow-water form (FP64 face fluxes from FP64 base fluxes and
+// FP32 deltas), a kernel of its own so the ideal-gas one compiles without it
+template <int dir, bool SW>
 __global__ void __launch_bounds__(BLK) k_flux_h_mix(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    flux_h_mix_body<dir>(g, st, lb);
+    flux_h_mix_body<dir, SW>(g, st, lb);
   }
 }
 
@@ -1766,7 +1767,7 @@ namespace {
 // launch kinds, each with its own block-prefix table over the instances
 enum {
   K_PRIM = 0, K_GHOST_U, K_GHOST_S, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX,
-  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_NKIND
+  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_FACE_MIX_SW, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -1875,7 +1876,8 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     case K_EDGE: return !mixed || !ideal ? (int)grid1((size_t)ned * g.nx1) : 0;
     case K_EDGE_MIX: return mixed && ideal ? (int)grid1((size_t)ned * g.nx1) : 0;
     case K_FACE: return !mixed ? (int)grid1(faces) : 0;
-    case K_FACE_MIX: return mixed ? (int)grid1(faces) : 0;
+    case K_FACE_MIX: return mixed && ideal ? (int)grid1(faces) : 0;
+    case K_FACE_MIX_SW: return mixed && !ideal ? (int)grid1(faces) : 0;
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
@@ -2119,8 +2121,10 @@ void launch_fluxes(Launcher &L) {
   launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
-  launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1>, BLK, 0);
-  launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2>, BLK, 0);
+  launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1, false>, BLK, 0);
+  launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2, false>, BLK, 0);
+  launch(L, K_FACE_MIX_SW, KID_FLUX_X2, k_flux_h_mix<1, true>, BLK, 0);
+  launch(L, K_FACE_MIX_SW, KID_FLUX_X3, k_flux_h_mix<2, true>, BLK, 0);
   HIPCHK(hipGetLastError());
 }
 
@@ -2184,7 +2188,8 @@ void tune_launcher(Launcher &L, int reps) {
   HIPCHK(hipEventCreate(&e1));
   bool prof = L.prof;
   L.prof = false;
-  const int kinds[] = {K_PRIM, K_GHOST_U, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX, K_DT};
+  const int kinds[] = {K_PRIM,     K_GHOST_U,  K_X1,          K_X1_MIX, K_EDGE,
+                       K_EDGE_MIX, K_FACE,     K_FACE_MIX,    K_FACE_MIX_SW, K_DT};
   for (int kind : kinds) {
     if (L.nblk[kind] == 0) continue;
     std::vector<int> cands = {0};
@@ -2204,7 +2209,8 @@ void tune_launcher(Launcher &L, int reps) {
           case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
           case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
           case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
-          case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1>, BLK, 0); break;
+          case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1, false>, BLK, 0); break;
+          case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
           case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
         }
       }
