@@ -229,13 +229,16 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
   double wv[5] = {0., 0., 0., 0., 0.};
   if (g.eos == EOS_SW) {
     double ud = st.u[g.c(IDN, p, k, i, j)];
-    ud = (ud > dfloor) ? ud : dfloor;
-    st.u[g.c(IDN, p, k, i, j)] = ud;
+    if (!(ud > dfloor)) {  // the floor, written back only where it applies
+      ud = dfloor;
+      st.u[g.c(IDN, p, k, i, j)] = ud;
+    }
     double v = st.u[g.c(IVY, p, k, i, j)], ww = st.u[g.c(IVZ, p, k, i, j)];
     double di = 1. / ud;
-    double sth2 = 1. - cth * cth;
-    double wy = v / sth2 - ww * cth / sth2;
-    double wz = -v * cth / sth2 + ww / sth2;
+    // the four quotients by sth2 share one correctly rounded reciprocal
+    double sth2 = 1. - cth * cth, ys = 1. / sth2;
+    double wy = div_rn(v, sth2, ys) - div_rn(ww * cth, sth2, ys);
+    double wz = div_rn(-v * cth, sth2, ys) + div_rn(ww, sth2, ys);
     wv[IDN] = ud;
     wv[IVX] = 0.;
     wv[IVY] = wy * di;
@@ -249,10 +252,10 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
     density += u_d;
     double di = 1. / density;
     double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
-    double sth2 = 1. - cth * cth;
+    double sth2 = 1. - cth * cth, ys = 1. / sth2;
     double v = w_vy, ww = w_vz;
-    w_vy = v / sth2 - ww * cth / sth2;
-    w_vz = -v * cth / sth2 + ww / sth2;
+    w_vy = div_rn(v, sth2, ys) - div_rn(ww * cth, sth2, ys);
+    w_vz = div_rn(-v * cth, sth2, ys) + div_rn(ww, sth2, ys);
     double KE = 0.5 * (u_m1 * w_vx + u_m2 * w_vy + u_m3 * w_vz);
     wv[IDN] = density;
     wv[IVX] = w_vx;
@@ -704,6 +707,9 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   double a2l = g.area2(p, k, j, i), a2r = g.area2(p, k, j + 1, i);
   double a3l = g.area3(p, k, j, i), a3r = g.area3(p, k + 1, j, i);
   double vol = g.volume(p, k, j, i);
+  // every quotient by vol (and by the radius below) shares one correctly
+  // rounded reciprocal: div_rn gives the IEEE quotient
+  double yvol = 1. / vol;
   // the first stage only sets u1 = delta u (u itself is unchanged when delta
   // is 1), later stages keep u1 unless delta != 0: arrays that would be read
   // only to be overwritten, or rewritten unchanged, are not touched
@@ -754,34 +760,35 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     if (tend) {
       if (!keep_u) st.u[c] = u;
       out[n] = 0.;
-      out[n] -= wght * dflx / vol;
+      out[n] -= div_rn(wght * dflx, vol, yvol);
     } else {
       out[n] = u;
-      out[n] -= wght * dflx / vol;
+      out[n] -= div_rn(wght * dflx, vol, yvol);
     }
   }
   // coordinate source terms
   double cosine = g.T(T_COSC, p, k, j), sinc = g.T(T_SINC, p, k, j);
   double sine2 = sinc * sinc;
-  double v1 = st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
+  // the shallow-water x1 velocity is zero for good (not loaded)
+  double v1 = g.eos == EOS_SW ? 0. : st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
   double v3 = st.w[g.c(IVZ, p, k, i, j)];
   double rho = st.w[g.c(IDN, p, k, i, j)];
-  double radius = g.x1v[i];
+  double radius = g.x1v[i], yrad = 1. / radius;
   double v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
   double pr;
   if (g.eos == EOS_SW) {
     pr = 0.5 * rho * rho;
   } else {
     pr = st.w[g.c(IPR, p, k, i, j)];
-    double src1 = (2.0 * pr + rho * (v2 * v_2 + v3 * v_3)) / radius;
+    double src1 = div_rn(2.0 * pr + rho * (v2 * v_2 + v3 * v_3), radius, yrad);
     out[IVX] += wght * src1;
   }
   // x_ov_rD / y_ov_rC of the constructor (gnomonic_equiangle.cpp:209-218)
-  double xovrD = (a2l * g.T(T_SIN2, p, k, j) - a2r * g.T(T_SIN2, p, k, j + 1)) / vol;
-  double yovrC = (a3l * g.T(T_SIN3, p, k, j) - a3r * g.T(T_SIN3, p, k + 1, j)) / vol;
-  double src2 = -xovrD * (pr + rho * v3 * v3 * sine2) - rho * v1 * v_2 / radius;
+  double xovrD = div_rn(a2l * g.T(T_SIN2, p, k, j) - a2r * g.T(T_SIN2, p, k, j + 1), vol, yvol);
+  double yovrC = div_rn(a3l * g.T(T_SIN3, p, k, j) - a3r * g.T(T_SIN3, p, k + 1, j), vol, yvol);
+  double src2 = -xovrD * (pr + rho * v3 * v3 * sine2) - div_rn(rho * v1 * v_2, radius, yrad);
   out[IVY] += wght * src2;
-  double src3 = -yovrC * (pr + rho * v2 * v2 * sine2) - rho * v1 * v_3 / radius;
+  double src3 = -yovrC * (pr + rho * v2 * v2 * sine2) - div_rn(rho * v1 * v_3, radius, yrad);
   out[IVZ] += wght * src3;
   // sources
   if (g.eos == EOS_IDEAL && g.grav != 0.0) {
@@ -846,7 +853,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       dflx += a2r * (double)S1[g.f(n, p, k, i, j + 1)] - a2l * (double)S1[g.f(n, p, k, i, j)];
       dflx += a3r * (double)S2[g.f(n, p, k + 1, i, j)] - a3l * (double)S2[g.f(n, p, k, i, j)];
     }
-    s -= wght * dflx / vol;
+    s -= div_rn(wght * dflx, vol, yvol);
     st.s[c] = s;
   }
 }
