@@ -58,7 +58,7 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
         "flux_x3": nv * 8 + nv * f + nt * (8 + f),
         # read u, u1, w, F1, F2, F3 (+ psf); write u, u1, du; scalars s, s1, G1..3
         "update": 6 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f),
-        "implicit": 5 * nv * 8,     # read du, w, u; write du, u
+        "implicit": 4 * nv * 8,     # read du, w, u; write u
         "prim": 2 * nv * 8 + 2 * nt * 8,  # read u (s), write w (r)
         "dt": nv * 8,               # read w
     }.get(kernel)
@@ -66,7 +66,8 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
 
 # kernel of the live profile -> its name in the rocprofv3 PMC summary
 PMC_NAMES = {"implicit": "k_implicit", "update": "k_update<true>", "flux_x1": "k_flux_x1_mix",
-             "flux_x2": "k_flux_h_mix<1>", "flux_x3": "k_flux_h_mix<2>", "prim": "k_prim"}
+             "flux_x2": "k_flux_h_mix<1, false>", "flux_x3": "k_flux_h_mix<2, false>",
+             "prim": "k_prim"}
 
 
 def measured_traffic(kernel):
