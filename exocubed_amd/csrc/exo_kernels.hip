@@ -370,8 +370,19 @@ __device__ __forceinline__ void psf_scan(const DGrid &g, const DState &st, int p
   double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
   double RdTv = pie / die;
   PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * RdTv));
-  for (int i = ie; i >= is; --i)
-    PSF(i) = PSF(i + 1) + grav * st.w[g.c(IDN, p, k, i, j)] * g.dx1f[i];
+  // the densities of eight levels are loaded before the recurrence uses them
+  constexpr int SB = 8;
+  for (int i0 = ie; i0 >= is; i0 -= SB) {
+    double rho[SB];
+#pragma unroll
+    for (int u = 0; u < SB; ++u)
+      rho[u] = i0 - u >= is ? st.w[g.c(IDN, p, k, i0 - u, j)] : 0.;
+#pragma unroll
+    for (int u = 0; u < SB; ++u) {
+      int i = i0 - u;
+      if (i >= is) PSF(i) = PSF(i + 1) + grav * rho[u] * g.dx1f[i];
+    }
+  }
   for (int q = 1; q <= NG; ++q) PSF(is - q) = PSF(is + q);
   for (int q = 1; q <= NG; ++q) PSF(ie + q + 1) = PSF(ie + 1 - q);
 #undef PSF
@@ -1448,6 +1459,12 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
   float gamma = (float)g.gamma;
   for (int fi = f0; fi < f1; ++fi) {
     float wl[5], wr[5], fl[5];
+    // the next window cell is loaded before this face is worked on
+    bool more = fi + 1 < f1;
+    double nxt[5];
+    if (more)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) nxt[n] = st.w[g.c(n, p, k, fi + 3, j)];
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       wl[n] = weno5f(q[4][n], q[3][n], q[2][n], q[1][n], q[0][n]);
@@ -1466,7 +1483,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
     riemann_pert(g.eos, g.rs, IVX, gamma, wl, wr, P, fl, cth, 0.f, 1.f, cosc);
     for (int n = 0; n < 5; ++n) st.fflx[0][g.f(n, p, k, fi, j)] = fl[n];
     st.psf[g.f(0, p, k, fi, j)] = pf;
-    if (fi + 1 < f1) {
+    if (more) {
 #pragma unroll
       for (int s = 0; s < 5; ++s)
 #pragma unroll
@@ -1478,11 +1495,11 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
       rsf[1] = rsf[2];
       rsf[2] = rsf[3];
       rsf[3] = rhi;
-      q[5][IDN] = (float)(st.w[g.c(IDN, p, k, i, j)] - rsv);
-      q[5][IVX] = (float)st.w[g.c(IVX, p, k, i, j)];
-      q[5][IVY] = (float)st.w[g.c(IVY, p, k, i, j)];
-      q[5][IVZ] = (float)st.w[g.c(IVZ, p, k, i, j)];
-      q[5][IPR] = (float)(st.w[g.c(IPR, p, k, i, j)] - psv);
+      q[5][IDN] = (float)(nxt[IDN] - rsv);
+      q[5][IVX] = (float)nxt[IVX];
+      q[5][IVY] = (float)nxt[IVY];
+      q[5][IVZ] = (float)nxt[IVZ];
+      q[5][IPR] = (float)(nxt[IPR] - psv);
     }
   }
   for (int n = 0; n < g.nt; ++n)
