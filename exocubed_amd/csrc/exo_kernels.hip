@@ -2466,8 +2466,9 @@ int exo_download_flux(exo_handle *h, int dir, double *out) {
   return guarded(h, [&] {
     if (dir < 0 || dir > 2 || !out) throw std::string("bad flux direction");
     if (h->mixed)
-      throw std::string("the mixed form keeps FP32 perturbation fluxes only; "
-                        "download fluxes from an EXO_PREC_FP64 handle");
+      throw std::string("the mixed form's face fluxes are not the reference's FP64 fluxes "
+                        "(FP32 perturbation or delta parts); download fluxes from an "
+                        "EXO_PREC_FP64 handle");
     Grid &g = h->g;
     std::vector<double> buf((size_t)g.nv * g.fvstride);
     HIPCHK(hipMemcpyAsync(buf.data(), h->st.flx[dir], buf.size() * sizeof(double),

@@ -115,6 +115,32 @@ def test_mixed_unstable_shallow_water(ex):
     assert np.all((e <= TOL_L2) | (e <= 4.0 * floor)), (e, floor)
 
 
+@pytest.mark.parametrize("rsolver", [0, 1])
+def test_mixed_shallow_water_tracers(ex, rsolver):
+    """Passive scalars on mixed shallow water: their face fluxes take the FP64
+    face mass flux; hydro and scalars within the bar, hydro mass conserved."""
+    cfg = sw_config(ex, nx=16, rsolver=rsolver, nx1=1)
+    cfg.ntracer = 2
+    u0, _ = ex.init_state(cfg, 0, 20261202, ())
+    s0 = np.random.default_rng(7).uniform(0.2, 1.0, (6, 2) + u0.shape[2:]) * u0[:, :1]
+    out = []
+    for prec in (ex.PREC_FP64, ex.PREC_MIXED):
+        cfg.precision = prec
+        g = ex.Solver(cfg)
+        g.upload(u0, s0)
+        m0 = g.sums()[0]
+        g.step(20, 0.0)
+        out.append((g.download(), abs(g.sums()[0] - m0) / abs(m0)))
+        g.close()
+    cfg.precision = ex.PREC_FP64
+    ((ua, sa), _), ((ub, sb), drift) = out
+    eu = rel_l2(interior(ub, 1), interior(ua, 1))[[0, 2, 3]]
+    es = rel_l2(interior(sb, 1), interior(sa, 1))
+    print("hydro", eu, "scalars", es, "mass drift", drift)
+    assert eu.max() <= TOL_L2 and es.max() <= TOL_L2, (eu, es)
+    assert drift <= 1e-13
+
+
 def test_mixed_tracers(ex):
     cfg = atm_config(ex, nx=12, nx1=12, rsolver=2, ntracer=2, coriolis=2)
     (a, b), _ = _pair(ex, cfg, 2, 10)
