@@ -212,44 +212,40 @@ __device__ __forceinline__ int xcd_order(int h, int n) {
 // ghost it mirrors into (ghost is - q mirrors is + q - 1, ghost ie + q mirrors
 // ie - q + 1).  With nx1 < NG the ghosts beyond the mirror of an interior cell
 // are not written (only shallow water allows it, and it never reads x1 ghosts).
-__device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int lb, double dfloor,
-                                          double sfloor) {
-  size_t t = (size_t)lb * BLK + threadIdx.x;
-  int ni = g.nx1;
-  size_t total = (size_t)6 * g.nc3 * ni * g.nc2;
-  if (t >= total) return;
-  int j = (int)(t % g.nc2);
-  size_t rr0 = t / g.nc2;
-  int i = g.is + (int)(rr0 % ni);
-  rr0 /= ni;
-  int k = (int)(rr0 % g.nc3), p = (int)(rr0 / g.nc3);
-  bool kin = k >= g.ks && k <= g.ke, jin = j >= g.js && j <= g.je;
-  if (!kin && !jin) return;  // corner ghosts are never read
+// Primitive state of one cell (p, k, i, j) from its conserved values uv
+// (EquationOfState::ConservedToPrimitive of shallow_yz_hydro.cpp /
+// ideal_moist_hydro.cpp): w, the reflecting x1 mirrors and the scalar mixing
+// ratios; the shallow-water depth floor is written back to u.
+__device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int p, int k, int i,
+                                          int j, const double *uv, double dfloor, double sfloor) {
   double cth = g.T(T_COSC, p, k, j);
   double wv[5] = {0., 0., 0., 0., 0.};
+  double rho_c;
   if (g.eos == EOS_SW) {
-    double ud = st.u[g.c(IDN, p, k, i, j)];
+    double ud = uv[IDN];
     if (!(ud > dfloor)) {  // the floor, written back only where it applies
       ud = dfloor;
       st.u[g.c(IDN, p, k, i, j)] = ud;
     }
-    double v = st.u[g.c(IVY, p, k, i, j)], ww = st.u[g.c(IVZ, p, k, i, j)];
+    double v = uv[IVY], ww = uv[IVZ];
     double di = 1. / ud;
     // the four quotients by sth2 share one correctly rounded reciprocal
     double sth2 = 1. - cth * cth, ys = 1. / sth2;
     double wy = div_rn(v, sth2, ys) - div_rn(ww * cth, sth2, ys);
     double wz = div_rn(-v * cth, sth2, ys) + div_rn(ww, sth2, ys);
+    rho_c = ud;
     wv[IDN] = ud;
     wv[IVX] = 0.;
     wv[IVY] = wy * di;
     wv[IVZ] = wz * di;
   } else {
     double gm1 = g.gamma - 1.0;
-    double u_d = st.u[g.c(IDN, p, k, i, j)], u_m1 = st.u[g.c(IVX, p, k, i, j)];
-    double u_m2 = st.u[g.c(IVY, p, k, i, j)], u_m3 = st.u[g.c(IVZ, p, k, i, j)];
-    double u_e = st.u[g.c(IEN, p, k, i, j)];
+    double u_d = uv[IDN], u_m1 = uv[IVX];
+    double u_m2 = uv[IVY], u_m3 = uv[IVZ];
+    double u_e = uv[IEN];
     double density = 0.;
     density += u_d;
+    rho_c = u_d;
     double di = 1. / density;
     double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
     double sth2 = 1. - cth * cth, ys = 1. / sth2;
@@ -275,7 +271,7 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
     if (mirror && qt <= NG) st.w[g.c(n, p, k, g.ie + qt, j)] = m;
   }
   for (int n = 0; n < g.nt; ++n) {
-    double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / st.u[g.c(IDN, p, k, i, j)]);
+    double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / rho_c);
     rr = rr < sfloor ? sfloor : rr;
     st.r[g.c(n, p, k, i, j)] = rr;
     if (qb <= NG) st.r[g.c(n, p, k, g.is - qb, j)] = rr;
@@ -283,10 +279,65 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
   }
 }
 
+
+__device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int lb, double dfloor,
+                                          double sfloor) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int ni = g.nx1;
+  size_t total = (size_t)6 * g.nc3 * ni * g.nc2;
+  if (t >= total) return;
+  int j = (int)(t % g.nc2);
+  size_t rr0 = t / g.nc2;
+  int i = g.is + (int)(rr0 % ni);
+  rr0 /= ni;
+  int k = (int)(rr0 % g.nc3), p = (int)(rr0 / g.nc3);
+  bool kin = k >= g.ks && k <= g.ke, jin = j >= g.js && j <= g.je;
+  if (!kin && !jin) return;  // corner ghosts are never read
+  double uv[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
+  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
+}
+
+// The ghost strips only (3 deep along each panel edge, corners excluded): the
+// primitive state of the interior cells of shallow water is written by the
+// stage's update.  Threads cover [p][side][depth][along][i].
+__device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st, int lb,
+                                                double dfloor, double sfloor) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  int N = g.N, ni = g.nx1;
+  size_t total = (size_t)6 * 4 * NG * N * ni;
+  if (t >= total) return;
+  int a = (int)(t % N);
+  size_t r = t / N;
+  int i = g.is + (int)(r % ni);
+  r /= ni;
+  int dq = (int)(r % NG);
+  r /= NG;
+  int side = (int)(r % 4), p = (int)(r / 4);
+  int k, j;
+  if (side == 0) { j = g.js - 1 - dq; k = g.ks + a; }
+  else if (side == 1) { j = g.je + 1 + dq; k = g.ks + a; }
+  else if (side == 2) { k = g.ks - 1 - dq; j = g.js + a; }
+  else { k = g.ke + 1 + dq; j = g.js + a; }
+  double uv[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
+  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
+}
+
 __global__ void __launch_bounds__(BLK) k_prim(const Inst *__restrict__ I_, Work w, double dfloor, double sfloor) {
   WORK_LOOP(w) {
     WORK_INST(w);
     prim_body(g, st, lb, dfloor, sfloor);
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_prim_ghost(const Inst *__restrict__ I_, Work w, double dfloor,
+                                                    double sfloor) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    prim_ghost_body(g, st, lb, dfloor, sfloor);
   }
 }
 
@@ -697,6 +748,8 @@ __global__ void __launch_bounds__(BLK) k_edge_flux(const Inst *__restrict__ I_, 
 struct StageW {
   double delta, g1, g2, beta;
   int first;
+  int prim;  // shallow water: also write the primitive state of the cell
+  double dfloor, sfloor;
 };
 
 // One cell (p, k, i, j) of the stage: the weighted average, AddFluxDivergence,
@@ -885,6 +938,9 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   ADDR(1) double *dst = g.eos == EOS_IDEAL ? st.du : st.u;
   for (int n = 0; n < g.nv; ++n)
     if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
+  // shallow water within a step: the stage's new state is final here, so its
+  // primitive state is written now (k_prim_ghost then covers the ghosts)
+  if (sw.prim && g.eos == EOS_SW) prim_from(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor);
 }
 
 template <bool MIX>
@@ -1791,7 +1847,10 @@ namespace {
 // launch kinds, each with its own block-prefix table over the instances
 enum {
   K_PRIM = 0, K_GHOST_U, K_GHOST_S, K_X1, K_X1_MIX, K_EDGE, K_EDGE_MIX, K_FACE, K_FACE_MIX,
-  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_FACE_MIX_SW, K_NKIND
+  K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_FACE_MIX_SW,
+  // within a step: the full primitive conversion for the ideal gas, the ghost
+  // strips for shallow water (its stage update writes the interior)
+  K_PRIM_REST, K_PRIM_GHOST, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -1891,6 +1950,8 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
   size_t cols = (size_t)g.N * g.N;
   switch (kind) {
     case K_PRIM: return (int)grid1((size_t)6 * g.nc3 * g.nx1 * g.nc2);
+    case K_PRIM_REST: return ideal ? (int)grid1((size_t)6 * g.nc3 * g.nx1 * g.nc2) : 0;
+    case K_PRIM_GHOST: return !ideal ? (int)grid1((size_t)6 * 4 * NG * g.N * g.nx1) : 0;
     case K_GHOST_U: return (int)grid1((size_t)ngd * g.nx1);
     case K_GHOST_S: return g.nt ? (int)grid1((size_t)ngd * g.nx1) : 0;
     case K_X1: return ideal && !mixed ? (int)grid1(cols, 64) * 6 : 0;
@@ -2157,9 +2218,12 @@ const double SS_G1[3] = {0.0, 0.25, 2.0 / 3.0};
 const double SS_G2[3] = {1.0, 0.75, 1.0 / 3.0};
 const double SS_BETA[3] = {1.0, 0.25, 2.0 / 3.0};
 
-void launch_update(Launcher &L, int stage) {
+// prim: the stage writes the primitive state of shallow-water cells (within a
+// step; the separate exo_add_flux_divergence call leaves it to exo_fill_ghosts)
+void launch_update(Launcher &L, int stage, int prim) {
   int st0 = stage - 1;
-  StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0};
+  StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
+            prim, DFLOOR, SFLOOR};
   launch(L, K_UPD, KID_UPDATE, k_update<false>, BLK, 0, sw);
   launch(L, K_UPD_MIX, KID_UPDATE, k_update<true>, BLK, 0, sw);
   HIPCHK(hipGetLastError());
@@ -2180,11 +2244,21 @@ void launch_dt(Launcher &L, double dt_fixed) {
   HIPCHK(hipGetLastError());
 }
 
+// ghost fill within a step: the stage update has written the primitive state
+// of the interior cells of shallow-water instances
+void launch_fill_ghosts_stage(Launcher &L) {
+  launch(L, K_GHOST_U, KID_GHOST, k_ghost, BLK, 0, 0);
+  launch(L, K_GHOST_S, KID_GHOST, k_ghost, BLK, 0, 1);
+  launch(L, K_PRIM_REST, KID_PRIM, k_prim, BLK, 0, DFLOOR, SFLOOR);
+  launch(L, K_PRIM_GHOST, KID_PRIM, k_prim_ghost, BLK, 0, DFLOOR, SFLOOR);
+  HIPCHK(hipGetLastError());
+}
+
 void run_stage(Launcher &L, int stage) {
   launch_fluxes(L);
-  launch_update(L, stage);
+  launch_update(L, stage, 1);
   launch_implicit(L, SS_BETA[stage - 1]);
-  launch_fill_ghosts(L);
+  launch_fill_ghosts_stage(L);
 }
 
 void run_step(Launcher &L, double dt_fixed) {
@@ -2489,7 +2563,7 @@ int exo_add_flux_divergence(exo_handle *h, double wght) {
     // stage-1 weights with beta * dt = wght: u1 <- u, u <- u1, then the update
     double dtv = wght;
     HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    launch_update(h->L, 1);
+    launch_update(h->L, 1, 0);
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
