@@ -1004,7 +1004,15 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   const ADDR(1) double *__restrict__ W = st.w;
   const ADDR(1) double *__restrict__ DU = st.du;
   ADDR(1) double *__restrict__ U = st.u;
-  ADDR(1) double *__restrict__ CF = st.coef;
+  // coefficient scratch, [block][level][pair][lane]: the 30 doubles of a level
+  // (a^-1 c row-major, then delta) as 15 pairs, each pair 64 lanes x 16 B
+  // contiguous, so a level is 15 coalesced 16-byte stores, and 15 loads back
+  typedef double dpair __attribute__((ext_vector_type(2)));
+  ADDR(1) dpair *__restrict__ CFP =
+      (ADDR(1) dpair *)st.coef + (size_t)lb * g.nx1 * 15 * 64 + threadIdx.x;
+  auto cfp = [&](int i, int q2) -> ADDR(1) dpair * {
+    return CFP + ((size_t)(i - is) * 15 + q2) * 64;
+  };
   size_t vs = g.vstride, jp = g.jp;
   size_t c0 = g.c(0, p, k, 0, j);  // column base (variable 0, i = 0)
   auto wload = [&](int i, double *w) {
@@ -1121,12 +1129,17 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
 #pragma unroll
       for (int n = 0; n < 5; ++n) rnext[n] = div_rn(dun[n], dt, ydt);
     }
+    {
+      double cv[30];
 #pragma unroll
-    for (int r = 0; r < 5; ++r)
+      for (int r = 0; r < 5; ++r)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)i * jp] = Aprev.a[r][q];
+        for (int q = 0; q < 5; ++q) cv[r * 5 + q] = Aprev.a[r][q];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) CF[(size_t)(25 + n) * vs + c0 + (size_t)i * jp] = delta.a[n];
+      for (int n = 0; n < 5; ++n) cv[25 + n] = delta.a[n];
+#pragma unroll
+      for (int q2 = 0; q2 < 15; ++q2) *cfp(i, q2) = dpair{cv[2 * q2], cv[2 * q2 + 1]};
+    }
     Am = Ap;
     em = e0;
     e0 = ep;
@@ -1144,37 +1157,45 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
     size_t cc = n * vs + c0 + (size_t)ie * jp;
     U[cc] += dnext.a[n];
   }
-  M5 An;
-  V5 dn, un;
-  if (ie - 1 >= is) {  // the coefficients and state of level ie - 1, one level ahead
+  // levels are loaded three ahead (a ring of three: 60 loads in flight at most)
+  constexpr int PD = 3;
+  M5 Aq[PD];
+  V5 dq[PD], uq[PD];
+  auto bload = [&](int l, M5 &A, V5 &d, V5 &u) {
+    double cv[30];
+#pragma unroll
+    for (int q2 = 0; q2 < 15; ++q2) {
+      dpair t = *cfp(l, q2);
+      cv[2 * q2] = t.x;
+      cv[2 * q2 + 1] = t.y;
+    }
 #pragma unroll
     for (int r = 0; r < 5; ++r)
 #pragma unroll
-      for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(ie - 1) * jp];
+      for (int q = 0; q < 5; ++q) A.a[r][q] = cv[r * 5 + q];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(ie - 1) * jp];
+    for (int n = 0; n < 5; ++n) d.a[n] = cv[25 + n];
 #pragma unroll
-    for (int n = 0; n < 5; ++n) un.a[n] = U[n * vs + c0 + (size_t)(ie - 1) * jp];
-  }
-  for (int i = ie - 1; i >= is; --i) {
-    M5 A = An;
-    V5 d = dn, uo = un;
-    if (i - 1 >= is) {  // the next level's coefficients and state, one level ahead
+    for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
+  };
 #pragma unroll
-      for (int r = 0; r < 5; ++r)
+  for (int q = 0; q < PD; ++q)
+    if (ie - 1 - q >= is) bload(ie - 1 - q, Aq[q], dq[q], uq[q]);
+  for (int i0 = ie - 1; i0 >= is; i0 -= PD) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) An.a[r][q] = CF[(size_t)(r * 5 + q) * vs + c0 + (size_t)(i - 1) * jp];
+    for (int q = 0; q < PD; ++q) {
+      int i = i0 - q;
+      if (i < is) break;
+      M5 A = Aq[q];
+      V5 d = dq[q], uo = uq[q];
+      if (i - PD >= is) bload(i - PD, Aq[q], dq[q], uq[q]);
+      V5 ad = m5_mulv(A, dnext);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) dn.a[n] = CF[(size_t)(25 + n) * vs + c0 + (size_t)(i - 1) * jp];
+      for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
 #pragma unroll
-      for (int n = 0; n < 5; ++n) un.a[n] = U[n * vs + c0 + (size_t)(i - 1) * jp];
+      for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = uo.a[n] + d.a[n];
+      dnext = d;
     }
-    V5 ad = m5_mulv(A, dnext);
-#pragma unroll
-    for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = uo.a[n] + d.a[n];
-    dnext = d;
   }
 }
 
@@ -2170,7 +2191,12 @@ void init_handle(exo_handle *h) {
       for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
     st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
   }
-  if (g.eos == EXO_EOS_IDEAL && g.implicit) st.coef = (double *)h->dalloc(h->cells_bytes(30));
+  if (g.eos == EXO_EOS_IDEAL && g.implicit) {
+    // k_implicit's [block][level][15 pairs][64 lanes] scratch within 30 cell arrays
+    size_t need = (size_t)6 * ((g.N * g.N + 63) / 64) * g.nx1 * 30 * 64;
+    if (need > (size_t)30 * g.vstride) throw std::string("implicit scratch layout does not fit");
+    st.coef = (double *)h->dalloc(h->cells_bytes(30));
+  }
   if (f32) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
     for (int d = 0; d < 3; ++d) {
