@@ -131,22 +131,52 @@ def run_gpu(ex, precision, steps, warmup, perturb=0.0):
     return dict(cells=cells, sec=sec, dt=dt, prof=prof, finite=finite, nv=5, u=u)
 
 
-def cpu_baseline(cfg_fn, ex):
-    from oracle.oracle import Oracle, lib
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    cores = int(os.environ.get("EXO_CPU_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    lib().or_set_threads(cores)
-    cfg = cfg_fn(ex, 0)
+
+def _oracle_rate(ex, cfg, steps):
+    from oracle.oracle import Oracle
     u0, _ = ex.init_state(cfg, 1, SEED, IC_PARAMS)
     o = Oracle(cfg)
     o.upload(u0)
     t0 = time.perf_counter()
-    o.step(1)
+    o.step(steps)
     sec = time.perf_counter() - t0
     o.close()
-    cells = 6 * cfg.nx * cfg.nx * cfg.nx1
-    return {"value": cells / sec, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"1 SSPRK3 step of the full 6x96x96x64 workload, {sec:.1f} s wall"}
+    return 6 * cfg.nx * cfg.nx * cfg.nx1 * steps / sec, sec
+
+
+def cpu_baseline(cfg_fn, ex, steps=4, small_nx=48):
+    """The C restatement (FP64, OpenMP over panel rows) timed on this host:
+    `steps` SSPRK3 steps of the full workload on all usable cores, then one
+    step of the same atmosphere on 6 x small_nx^2 x 64 cells on one thread."""
+    from oracle.oracle import lib
+
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = int(os.environ.get("EXO_CPU_THREADS", "0")) or min(16, avail or 1)
+    lib().or_set_threads(cores)
+    cfg = cfg_fn(ex, 0)
+    rate, sec = _oracle_rate(ex, cfg, steps)
+    lib().or_set_threads(1)
+    small = cfg_fn(ex, 0)
+    small.nx = small_nx
+    rate1, sec1 = _oracle_rate(ex, small, 1)
+    lib().or_set_threads(cores)
+    return {"value": rate, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} SSPRK3 steps of the full 6x96x96x64 workload on {cores} "
+                      f"threads, {sec:.1f} s wall",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "usable_cpus": avail,
+            "one_thread": {"value": rate1, "unit": "cell-updates/s", "cores": 1,
+                           "sample": f"1 SSPRK3 step of 6x{small_nx}x{small_nx}x64 (same "
+                                     f"atmosphere), {sec1:.1f} s wall"}}
 
 
 R_EARTH = 6371000.0
@@ -269,6 +299,47 @@ def large(ex, steps):
                         "with rotation, WENO5+HLLC, implicit 5x5 FP64, mixed form"}
 
 
+def output_bench(ex, reps=10):
+    """The state of the headline workload written out as the reference's
+    per-block NetCDF files: the device kernel that lays the float fields out
+    (live HIP-event time, algorithmic bytes: each FP64 input read once, each
+    float written once), the fields copied to host memory, and the six files."""
+    import tempfile
+
+    cfg = workload_config(ex, ex.PREC_FP64)
+    u0, _ = ex.init_state(cfg, 1, SEED, IC_PARAMS)
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    del u0
+    g.step(1)
+    cells = 6 * cfg.nx * cfg.nx * cfg.nx1
+    out = {}
+    for var, nin in (("prim", 5), ("uov", 2)):
+        nq = len(ex.output_names(cfg, var))
+        g.output_fields(var)
+        g.profile_begin()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f = g.output_fields(var)
+        t1 = time.perf_counter()
+        ms, n = g.profile_end()["output"]
+        bpc = nin * 8 + nq * 4
+        avg = ms / n
+        with tempfile.TemporaryDirectory() as d:
+            t2 = time.perf_counter()
+            g.write_netcdf(os.path.join(d, "out"), var, 0.0, 0)
+            t3 = time.perf_counter()
+            size = sum(os.path.getsize(os.path.join(d, x)) for x in os.listdir(d))
+        out[var] = {"fields": nq, "kernel_ms": avg, "bytes_per_cell": bpc,
+                    "kernel_GB_s": bpc * cells / (avg * 1e-3) / 1e9,
+                    "kernel_hbm_frac": bpc * cells / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "to_host_ms": (t1 - t0) * 1e3 / reps, "host_MB": f.nbytes / 1e6,
+                    "write_files_ms": (t3 - t2) * 1e3, "file_MB": size / 1e6}
+    g.close()
+    out["workload"] = "6x96x96x64 ideal-gas state after one step, interior cells, 6 files"
+    return out
+
+
 def serving(ex, precision, nreq, keep=2):
     """Poisson request stream through the continuous-batching server."""
     rng = np.random.default_rng(SEED + 3)
@@ -387,6 +458,7 @@ def main():
                "workload": "Poisson 40 req/s, 3D ideal gas + 2 tracers, N 32..128, 32..96 "
                            "levels, WENO5+LMARS, implicit 5x5, 200 steps each"}
     big = large(ex, args.large_steps) if args.large_steps > 0 and not args.only_headline else None
+    outp = None if args.only_headline else output_bench(ex)
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
@@ -417,6 +489,7 @@ def main():
         "ensemble": ens,
         "serving": srv,
         "large": big,
+        "output": outp,
     }
     print(json.dumps(out))
 

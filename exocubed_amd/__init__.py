@@ -21,7 +21,7 @@ COR_NONE, COR_FORCING2, COR_3D = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 BATCH_GRAPH = 1
 KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "update",
-           "implicit", "dt")
+           "implicit", "dt", "output")
 # launch kinds of the persistent-width tables (exo_tune / exo_batch_tune)
 LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
                 "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one", "dt",
@@ -116,6 +116,14 @@ def lib():
         "exo_server_stats": (C.c_int, [H, C.c_void_p, C.c_int]),
         "exo_server_last_error": (C.c_char_p, [H]),
         "exo_server_destroy": (C.c_int, [H]),
+        "exo_output_names": (C.c_int, [C.POINTER(Config), C.c_char_p, C.c_char_p, C.c_size_t,
+                                       C.POINTER(C.c_int)]),
+        "exo_output_fields": (C.c_int, [H, C.c_char_p, C.c_int, C.c_void_p, C.c_size_t]),
+        "exo_write_netcdf": (C.c_int, [H, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
+                                       C.c_double, C.c_int, C.c_int]),
+        "exo_write_netcdf_fields": (C.c_int, [C.POINTER(Config), C.c_char_p, C.c_char_p,
+                                              C.c_char_p, C.c_int, C.c_double, C.c_int,
+                                              C.c_int, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -158,6 +166,43 @@ def describe(cfg: Config, which: int) -> np.ndarray:
                       C.byref(n)) != 0:
         raise RuntimeError("exo_describe failed")
     return out
+
+
+def output_names(cfg: Config, variable: str):
+    """Field names of an output variable ("prim", "cons", "uov") in file order."""
+    L = lib()
+    buf = C.create_string_buffer(1024)
+    n = C.c_int()
+    if L.exo_output_names(C.byref(cfg), variable.encode(), buf, len(buf), C.byref(n)) != 0:
+        raise ExoError(f"unknown output variable {variable!r}")
+    return buf.value.decode().split(",")
+
+
+def output_shape(cfg: Config, variable: str, ghost_zones=False):
+    g = 6 if ghost_zones else 0
+    return (6, len(output_names(cfg, variable)), cfg.nx1 + g, cfg.nx + g, cfg.nx + g)
+
+
+def write_netcdf_fields(cfg: Config, basename, variable, fields, time=0.0, file_number=0,
+                        file_id=None, blocks_per_side=1, ghost_zones=False):
+    """Per-block NetCDF files from host fields in the output layout (no GPU)."""
+    f = np.ascontiguousarray(fields, dtype=np.float32)
+    if f.shape != output_shape(cfg, variable, ghost_zones):
+        raise ExoError(f"fields of shape {f.shape}, expected "
+                       f"{output_shape(cfg, variable, ghost_zones)}")
+    rc = lib().exo_write_netcdf_fields(C.byref(cfg), os.fsencode(basename), variable.encode(),
+                                       None if file_id is None else file_id.encode(),
+                                       file_number, time, blocks_per_side, int(ghost_zones),
+                                       f.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise ExoError(f"exo_write_netcdf_fields failed ({rc})")
+
+
+def output_file_names(basename, variable, file_number=0, file_id=None, blocks_per_side=1):
+    """Names of the files one output writes, in gid order."""
+    fid = variable if file_id is None else file_id
+    return [f"{basename}.block{g}.{fid}.{file_number:05d}.nc"
+            for g in range(6 * blocks_per_side * blocks_per_side)]
 
 
 GHOST_DTYPE = np.dtype([(f, np.int32) for f in
@@ -255,8 +300,7 @@ class Solver:
         self._chk(self.L.exo_new_dt(self.h, C.byref(d)), "exo_new_dt")
         return d.value
 
-    KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "update",
-               "implicit", "dt")
+    KERNELS = KERNELS
 
     def profile_begin(self):
         self._chk(self.L.exo_profile_begin(self.h), "exo_profile_begin")
@@ -271,6 +315,23 @@ class Solver:
 
     def sync(self):
         self._chk(self.L.exo_sync(self.h), "exo_sync")
+
+    def output_fields(self, variable="prim", ghost_zones=False):
+        """Float fields of an output variable, [6][nvar][x1][x2][x3] (x3 fastest)."""
+        out = np.zeros(output_shape(self.cfg, variable, ghost_zones), dtype=np.float32)
+        self._chk(self.L.exo_output_fields(self.h, variable.encode(), int(ghost_zones),
+                                           out.ctypes.data_as(C.c_void_p), out.size),
+                  "exo_output_fields")
+        return out
+
+    def write_netcdf(self, basename, variable="prim", time=0.0, file_number=0, file_id=None,
+                     blocks_per_side=1, ghost_zones=False):
+        """The reference's per-block NetCDF files of the current state."""
+        self._chk(self.L.exo_write_netcdf(self.h, os.fsencode(basename), variable.encode(),
+                                          None if file_id is None else file_id.encode(),
+                                          file_number, time, blocks_per_side, int(ghost_zones)),
+                  "exo_write_netcdf")
+        return output_file_names(basename, variable, file_number, file_id, blocks_per_side)
 
     def sums(self):
         """Compensated FP64 totals sum(u_n * volume) per conserved variable."""
@@ -423,3 +484,30 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class NetcdfOutput:
+    """One ``<outputN>`` block of type netcdf: writes every ``dt`` of simulated
+    time, numbering the files from ``file_number`` (Athena's OutputType
+    cadence; src/outputs/netcdf.cpp:459-466 advances file_number and
+    next_time after each write)."""
+
+    def __init__(self, basename, variable="prim", dt=0.0, file_id=None, ghost_zones=False,
+                 blocks_per_side=1, file_number=0, next_time=0.0):
+        self.basename, self.variable, self.dt = basename, variable, dt
+        self.file_id, self.ghost_zones, self.blocks_per_side = file_id, ghost_zones, \
+            blocks_per_side
+        self.file_number, self.next_time = file_number, next_time
+
+    def due(self, time):
+        return time >= self.next_time
+
+    def write(self, solver: Solver, time):
+        files = solver.write_netcdf(self.basename, self.variable, time, self.file_number,
+                                    self.file_id, self.blocks_per_side, self.ghost_zones)
+        self.file_number += 1
+        self.next_time += self.dt
+        return files
+
+    def maybe_write(self, solver: Solver, time):
+        return self.write(solver, time) if self.due(time) else []

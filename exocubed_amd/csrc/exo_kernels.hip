@@ -19,6 +19,7 @@
 #include "../../include/exocubed.h"
 #include "exo_device.hpp"
 #include "exo_mixed.hpp"
+#include "exo_netcdf.hpp"
 #include "exo_setup.hpp"
 
 using namespace exo;
@@ -1858,6 +1859,83 @@ __global__ void k_math_batch(int op, const double *x, double y, double *out, int
   out[t] = op == 0 ? dexp(x[t]) : (op == 1 ? dlog(x[t]) : dpow(x[t], y));
 }
 
+// ---------------------------------------------------------------------------
+// output fields in the per-block file layout (src/outputs/netcdf.cpp:439-447:
+// float, x1 outermost, x3 fastest)
+constexpr int OUT_MAX = 24;
+enum { OP_COPY = 0, OP_TAB, OP_U, OP_V };
+struct OutMap {
+  const double *src[OUT_MAX];  // OP_COPY: the field's cell array; OP_TAB: a table plane
+  int op[OUT_MAX];
+  int flip[OUT_MAX];           // x1 vector component: negated in the x1 mirror
+};
+
+// out[p][q][i'][j'][k'] from the device layout [k][i][j]: one workgroup moves a
+// 64 x 64 (k, j) tile of one field at one x1 level through LDS, reading rows
+// of 64 doubles along x2 and writing rows of 64 floats along x3.
+//   * The lat-lon wind takes the contravariant (v2, v3) of the cell and its
+//     column's factors in the grouping of CubedSphere::GetUV
+//     (cubed_sphere.cpp:132-145).
+//   * x1 ghost levels are the reflecting mirrors of the interior (the x1
+//     component negated), as the reflecting boundary leaves w.
+//   * Ghost cells outside both the x2 and the x3 interior range (the panel
+//     corners, which no stencil reads and no exchange fills here) are NaN,
+//     except for the geometric fields.
+__global__ void __launch_bounds__(256) k_out_fields(Grid g, OutMap m, const double *wy,
+                                                    const double *wz, const double *uvt,
+                                                    int nq, int ghost, float *out) {
+  __shared__ float tile[64][65];
+  int G = ghost ? NG : 0;
+  int n1 = g.nx1 + 2 * G, n2 = g.N + 2 * G, n3 = n2;
+  int ntj = (n2 + 63) >> 6;
+  int tj = blockIdx.x % ntj, tk = blockIdx.x / ntj;
+  int i = blockIdx.y, p = blockIdx.z / nq, q = blockIdx.z % nq;
+  // the interior level an x1 ghost level mirrors: reflections repeated until
+  // inside (more than one only when nx1 < 3), the x1 component negated once
+  // per reflection
+  int isrc = g.is - G + i;
+  bool mir = false;
+  for (int r = 0; r < 4 && (isrc < g.is || isrc > g.ie); ++r) {
+    isrc = isrc < g.is ? 2 * g.is - 1 - isrc : 2 * g.ie + 1 - isrc;
+    mir = !mir;
+  }
+  int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+  int op = m.op[q];
+  size_t tp = (size_t)6 * g.nc3 * g.nc2;
+  for (int r = ly; r < 64; r += 4) {
+    int kk = tk * 64 + r, jj = tj * 64 + lx;
+    if (kk < n3 && jj < n2) {
+      int k = g.ks - G + kk, j = g.js - G + jj;
+      bool corner = (k < g.ks || k > g.ke) && (j < g.js || j > g.je);
+      size_t c = g.c(0, p, k, isrc, j);
+      size_t t = ((size_t)p * g.nc3 + k) * g.nc2 + j;
+      double v;
+      if (op == OP_TAB) {
+        v = m.src[q][t];
+      } else if (corner) {
+        v = __builtin_nan("");
+      } else if (op == OP_COPY) {
+        v = m.src[q][c];
+        if (mir && m.flip[q]) v = -v;
+      } else {
+        double V2 = wy[c], V3 = wz[c];
+        double f1 = uvt[3 * tp + t], f2 = uvt[4 * tp + t];
+        double f3 = uvt[5 * tp + t], f4 = uvt[6 * tp + t];
+        if (p == 0) v = op == OP_U ? -(f1 * V2 + f2 * V3) : f3 * V2 - f4 * V3;
+        else if (p == 4) v = op == OP_U ? -(f1 * V2 - f2 * V3) : f3 * V2 + f4 * V3;
+        else v = op == OP_U ? -(f1 * V3) : f2 * V3 - V2;
+      }
+      tile[r][lx] = (float)v;
+    }
+  }
+  __syncthreads();
+  for (int r = ly; r < 64; r += 4) {
+    int jj = tj * 64 + r, kk = tk * 64 + lx;
+    if (kk < n3 && jj < n2)
+      out[((((size_t)p * nq + q) * n1 + i) * n2 + jj) * n3 + kk] = tile[lx][r];
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1876,7 +1954,7 @@ enum {
 
 // kernel ids of the profile (exo_profile_end)
 enum { KID_PRIM = 0, KID_GHOST, KID_FLUX_X1, KID_EDGE, KID_FLUX_X2, KID_FLUX_X3, KID_UPDATE,
-       KID_IMPLICIT, KID_DT, KID_COUNT };
+       KID_IMPLICIT, KID_DT, KID_OUTPUT, KID_COUNT };
 
 unsigned grid1(size_t n, unsigned b = BLK) { return (unsigned)((n + b - 1) / b); }
 
@@ -1921,6 +1999,9 @@ struct exo_handle {
   EdgeDesc *d_edge = nullptr;
   double *d_x1 = nullptr, *d_tab = nullptr;
   double *d_sums = nullptr;     // [5] totals of exo_conserved_sums
+  double *d_uov = nullptr;      // per-column factors of the "uov" output (first use)
+  float *d_out = nullptr;       // output fields (grown on demand)
+  size_t out_cap = 0;           // floats in d_out
   bool have_state = false;
   bool mixed = false;
   // one SSPRK3 step captured as a HIP graph and replayed by exo_step
@@ -2471,6 +2552,72 @@ int guarded(T *h, F &&fn) {
   }
 }
 
+// Output sets: fields per cell, the device buffer they go to, the launch.
+size_t output_floats(const Grid &g, int set, int ghost) {
+  int G = ghost ? NG : 0;
+  size_t n1 = g.nx1 + 2 * G, n2 = g.N + 2 * G;
+  return (size_t)6 * out_names(set, g.eos, g.nt).size() * n1 * n2 * n2;
+}
+
+// writes the fields of one output set into h->d_out (stream-ordered after
+// every launch before it); returns the number of fields
+int output_fields_dev(exo_handle *h, int set, int ghost) {
+  Grid &g = h->g;
+  int nq = (int)out_names(set, g.eos, g.nt).size();
+  if (nq > OUT_MAX) throw std::string("too many output fields (at most 24)");
+  size_t need = output_floats(g, set, ghost);
+  if (need > h->out_cap) {
+    if (h->d_out) HIPCHK(hipFree(h->d_out));
+    h->d_out = nullptr;
+    h->out_cap = 0;
+    HIPCHK(hipMalloc(&h->d_out, need * sizeof(float)));
+    h->out_cap = need;
+  }
+  OutMap m{};
+  size_t tp = (size_t)6 * g.nc3 * g.nc2;
+  if (set == OUT_UOV) {
+    if (!h->d_uov) {
+      std::vector<double> t = uov_table(h->setup);
+      h->d_uov = (double *)h->dalloc(t.size() * sizeof(double));
+      HIPCHK(hipMemcpyAsync(h->d_uov, t.data(), t.size() * sizeof(double),
+                            hipMemcpyHostToDevice, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    const int ops[5] = {OP_TAB, OP_TAB, OP_U, OP_V, OP_TAB};
+    const double *src[5] = {h->d_uov, h->d_uov + tp, nullptr, nullptr, h->d_uov + 2 * tp};
+    for (int q = 0; q < 5; ++q) {
+      m.op[q] = ops[q];
+      m.src[q] = src[q];
+    }
+  } else {
+    const double *base = set == OUT_PRIM ? h->st.w : h->st.u;
+    const double *sbase = set == OUT_PRIM ? h->st.r : h->st.s;
+    int q = 0;
+    m.src[q++] = base + IDN * g.vstride;
+    if (g.eos == EOS_IDEAL) m.src[q++] = base + IPR * g.vstride;
+    for (int d = IVX; d <= IVZ; ++d) {
+      m.flip[q] = d == IVX;
+      m.src[q++] = base + d * g.vstride;
+    }
+    for (int t = 0; t < g.nt; ++t) m.src[q++] = sbase + t * g.vstride;
+    for (int e = 0; e < q; ++e) m.op[e] = OP_COPY;
+  }
+  int G = ghost ? NG : 0;
+  unsigned n2 = g.N + 2 * G, nt = (n2 + 63) / 64;
+  dim3 grid(nt * nt, g.nx1 + 2 * G, 6 * nq);
+  Launcher &L = h->L;
+  if (L.prof) {
+    L.ev_kind.push_back(KID_OUTPUT);
+    HIPCHK(hipEventRecord(next_event(L), L.stream));
+  }
+  k_out_fields<<<grid, dim3(256), 0, h->stream>>>(g, m, h->st.w + IVY * g.vstride,
+                                                   h->st.w + IVZ * g.vstride, h->d_uov, nq,
+                                                   ghost, h->d_out);
+  HIPCHK(hipGetLastError());
+  if (L.prof) HIPCHK(hipEventRecord(next_event(L), L.stream));
+  return nq;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2511,6 +2658,7 @@ int exo_destroy(exo_handle *h) {
   if (h->graph) (void)hipGraphDestroy(h->graph);
   for (hipEvent_t e : h->L.ev_pool) (void)hipEventDestroy(e);
   for (void *p : h->allocs) (void)hipFree(p);
+  if (h->d_out) (void)hipFree(h->d_out);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return EXO_OK;
@@ -2762,6 +2910,75 @@ int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, const dou
 // would upload.  which: 0 ghost descriptors, 1 edge descriptors, 2 metric
 // tables [T_NTAB][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f of the six panels,
 // 5 x3f of the six panels.
+// ---------------------------------------------------------------------------
+// output (NetcdfOutput::WriteOutputFile, src/outputs/netcdf.cpp:34-469)
+int exo_output_names(const exo_config *cfg, const char *variable, char *names, size_t cap,
+                     int *nvar) {
+  int set = out_set_from_name(variable);
+  if (!cfg || !nvar || set < 0) return EXO_ERR_ARG;
+  std::vector<std::string> n = out_names(set, cfg->eos, cfg->ntracer);
+  std::string joined;
+  for (auto &x : n) joined += (joined.empty() ? "" : ",") + x;
+  *nvar = (int)n.size();
+  if (names) {
+    if (cap < joined.size() + 1) return EXO_ERR_ARG;
+    std::memcpy(names, joined.c_str(), joined.size() + 1);
+  }
+  return EXO_OK;
+}
+
+int exo_output_fields(exo_handle *h, const char *variable, int ghost_zones, float *out,
+                      size_t cap_floats) {
+  return guarded(h, [&] {
+    int set = out_set_from_name(variable);
+    if (set < 0) throw std::string("unknown output variable (prim, cons or uov)");
+    if (!h->have_state) throw std::string("no state uploaded");
+    if (!out) throw std::string("null output");
+    size_t need = output_floats(h->g, set, ghost_zones != 0);
+    if (cap_floats < need)
+      throw std::string("output buffer too small: ") + std::to_string(need) + " floats needed";
+    output_fields_dev(h, set, ghost_zones != 0);
+    HIPCHK(hipMemcpyAsync(out, h->d_out, need * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_write_netcdf(exo_handle *h, const char *basename, const char *variable,
+                     const char *file_id, int file_number, double time, int blocks_per_side,
+                     int ghost_zones) {
+  return guarded(h, [&] {
+    int set = out_set_from_name(variable);
+    if (set < 0) throw std::string("unknown output variable (prim, cons or uov)");
+    if (!basename) throw std::string("null basename");
+    if (!h->have_state) throw std::string("no state uploaded");
+    size_t need = output_floats(h->g, set, ghost_zones != 0);
+    std::vector<float> buf(need);
+    output_fields_dev(h, set, ghost_zones != 0);
+    HIPCHK(hipMemcpyAsync(buf.data(), h->d_out, need * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::string m = write_output_files(h->setup, basename, set, file_id ? file_id : variable,
+                                       file_number, time, blocks_per_side, ghost_zones != 0,
+                                       buf.data());
+    if (!m.empty()) throw m;
+  });
+}
+
+int exo_write_netcdf_fields(const exo_config *cfg, const char *basename, const char *variable,
+                            const char *file_id, int file_number, double time,
+                            int blocks_per_side, int ghost_zones, const float *fields) {
+  int set = out_set_from_name(variable);
+  if (!cfg || !basename || !fields || set < 0) return EXO_ERR_ARG;
+  Setup s;
+  if (!build_setup(*cfg, &s).empty()) return EXO_ERR_ARG;
+  std::string m = write_output_files(s, basename, set, file_id ? file_id : variable, file_number,
+                                     time, blocks_per_side, ghost_zones != 0, fields);
+  if (m.empty()) return EXO_OK;
+  return m.rfind("cannot open", 0) == 0 || m.rfind("write error", 0) == 0 ? EXO_ERR_IO
+                                                                         : EXO_ERR_ARG;
+}
+
 int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t *count) {
   if (!cfg || !count) return EXO_ERR_ARG;
   Setup s;

@@ -37,6 +37,7 @@ extern "C" {
 #define EXO_ERR_STATE (-3)
 #define EXO_ERR_NOMEM (-4)
 #define EXO_ERR_UNSUPPORTED (-5)
+#define EXO_ERR_IO (-6)
 
 /* equation of state: cmake/exo2.cmake:14 (shallow_yz), parameters.cmake:18 */
 enum { EXO_EOS_SHALLOW_YZ = 0, EXO_EOS_IDEAL = 1 };
@@ -125,7 +126,7 @@ EXO_API int exo_sync(exo_handle *h);
 /* Live kernel timing: between begin and end every launch on the handle's stream
  * is bracketed by a HIP event pair; end returns the summed milliseconds and
  * launch counts per kernel id (0 prim, 1 ghost, 2 flux_x1, 3 edge_states,
- * 4 flux_x2, 5 flux_x3, 6 update, 7 implicit, 8 dt). */
+ * 4 flux_x2, 5 flux_x3, 6 update, 7 implicit, 8 dt, 9 output). */
 EXO_API int exo_profile_begin(exo_handle *h);
 EXO_API int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
 
@@ -244,6 +245,40 @@ EXO_API int exo_math_batch(exo_handle *h, int op, const double *x, double y, dou
  * With buf == NULL only *count is returned. */
 EXO_API int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap_bytes,
                  size_t *count);
+
+/* --- output: the reference's per-MeshBlock NetCDF files ------------------ */
+/* Replaces NetcdfOutput::WriteOutputFile (src/outputs/netcdf.cpp:34-469) for
+ * the output variables this path has:
+ *   "prim": rho, press (ideal gas), vel1, vel2, vel3, r0..r{NTRACER-1}
+ *   "cons": dens, Etot (ideal gas), mom1, mom2, mom3, s0..s{NTRACER-1}
+ *   "uov" : lat, lon, U, V, sqrtg, the user outputs of
+ *           examples/2023-Chen-exo3/W92.cpp:115-151 (MeshBlock::
+ *           UserWorkBeforeOutput; degrees, lat-lon wind, delta / (C D))
+ * Field names in file order, comma separated (host only, no GPU needed). */
+EXO_API int exo_output_names(const exo_config *cfg, const char *variable, char *names,
+                             size_t cap, int *nvar);
+/* The fields of one output variable of the current state as float, laid out
+ * as the files hold them: [6 panels][nvar][n1][n2][n3], x3 fastest, n the
+ * interior cells or (ghost_zones != 0) the cells with three ghosts on each
+ * side.  Computed on the device from the FP64 state of the last full stage. */
+EXO_API int exo_output_fields(exo_handle *h, const char *variable, int ghost_zones, float *out,
+                              size_t cap_floats);
+/* Writes <basename>.block<gid>.<file_id>.<file_number %05d>.nc for every
+ * MeshBlock: each panel split into blocks_per_side^2 blocks (a power of two
+ * dividing N), gid in Athena's MeshBlockTree order (panel, then Z-order of
+ * the block within it).  file_id NULL = the variable name (Athena's default
+ * "id").  NetCDF-3 64-bit-offset files (the reference asks libnetcdf for
+ * NetCDF-4); x3f holds x3 face coordinates (the reference copies x1f there,
+ * netcdf.cpp:306-310). */
+EXO_API int exo_write_netcdf(exo_handle *h, const char *basename, const char *variable,
+                             const char *file_id, int file_number, double time,
+                             int blocks_per_side, int ghost_zones);
+/* The same files from host fields in the exo_output_fields layout (host only,
+ * no GPU needed); EXO_ERR_IO when a file cannot be written. */
+EXO_API int exo_write_netcdf_fields(const exo_config *cfg, const char *basename,
+                                    const char *variable, const char *file_id, int file_number,
+                                    double time, int blocks_per_side, int ghost_zones,
+                                    const float *fields);
 
 #ifdef __cplusplus
 }

@@ -1818,6 +1818,11 @@ void or_get_scalars(const or_solver *s, double *sc) {
   for (int p = 0; p < 6; ++p) memcpy(sc + p * n4, s->pb[p].s, n4 * sizeof(double));
 }
 
+void or_get_scalar_prim(const or_solver *s, double *r) {
+  size_t n4 = (size_t)s->nt * s->nc3 * s->nc2 * s->nc1;
+  for (int p = 0; p < 6; ++p) memcpy(r + p * n4, s->pb[p].r, n4 * sizeof(double));
+}
+
 void or_get_prim(const or_solver *s, double *w) {
   size_t n4 = (size_t)s->nv * s->nc3 * s->nc2 * s->nc1;
   for (int p = 0; p < 6; ++p) memcpy(w + p * n4, s->pb[p].w, n4 * sizeof(double));
@@ -1865,6 +1870,45 @@ double or_face_area(const or_solver *s, int p, int dir, int k, int j, int i) {
 
 void or_latlon(const or_solver *s, int p, int k, int j, double *lat, double *lon) {
   cs_latlon(s, &s->pb[p], k, j, lat, lon);
+}
+
+/* MeshBlock::UserWorkBeforeOutput of examples/2023-Chen-exo3/W92.cpp:115-141
+ * at one point: lat and lon in degrees (CubedSphere::GetLatLon,
+ * cubed_sphere.cpp:78-90), the lat-lon wind U, V of the contravariant (Vy, Vz)
+ * (CubedSphere::GetUV, :132-145) and sqrtg = delta / (C D). */
+void or_w92_user_out_point(double x2v, double x3v, int blockID, double Vy, double Vz,
+                           double *out5) {
+  double x = tan(x2v);
+  double y = tan(x3v);
+  double C = sqrt(1.0 + x * x);
+  double D = sqrt(1.0 + y * y);
+  double delta = 1.0 / (1.0 + x * x + y * y);
+  double lat, lon, U, V;
+  rll_from_xyp(y, -x, blockID - 1, &lon, &lat);
+  lon = 2.0 * OR_PI - lon;
+  out5[0] = lat / OR_PI * 180.0;
+  out5[1] = lon / OR_PI * 180.0;
+  vec_rll_from_abp(x, y, blockID, Vy, Vz, &U, &V);
+  out5[2] = -U;
+  out5[3] = V;
+  out5[4] = delta / (C * D);
+}
+
+/* the five user outputs at every cell (ghosts included) of the current
+ * primitive state: out (6, 5, nc3, nc2, nc1) */
+void or_user_output(const or_solver *s, double *out) {
+  size_t n3 = (size_t)s->nc3 * s->nc2 * s->nc1;
+  for (int p = 0; p < 6; ++p) {
+    const or_panel *P = &s->pb[p];
+    for (int k = 0; k < s->nc3; ++k)
+      for (int j = 0; j < s->nc2; ++j)
+        for (int i = 0; i < s->nc1; ++i) {
+          double o[5];
+          or_w92_user_out_point(P->x2v[j], P->x3v[k], P->id, P->w[C4(IVY, k, j, i)],
+                                P->w[C4(IVZ, k, j, i)], o);
+          for (int q = 0; q < 5; ++q) out[(p * 5 + q) * n3 + C4(0, k, j, i)] = o[q];
+        }
+  }
 }
 
 /* ------------------------------------------------------------------------ */

@@ -13,7 +13,9 @@
  *   pinned by golden vectors : or_weno5 / or_weno3 (tests/test_weno.cpp:11-29)
  *   pinned by closed forms   : geometry (areas/volumes), vector transforms,
  *                              ghost remap of constant / smooth fields,
- *                              block-Thomas vs dense solve, Riemann consistency
+ *                              block-Thomas vs dense solve, Riemann consistency,
+ *                              W92 user outputs (known answers, rigid lat-lon
+ *                              map, wind speed and direction)
  *   parity unpinned          : Athena-side semantics not in the reference tree
  *                              (AddFluxDivergence, SSPRK3 weight order,
  *                              NewBlockTimeStep, reflecting BC fill) -- restated
@@ -69,6 +71,7 @@ void or_get_cons(const or_solver *s, double *u);
 void or_get_prim(const or_solver *s, double *w);
 void or_set_scalars(or_solver *s, const double *sc); /* conserved s = rho*r */
 void or_get_scalars(const or_solver *s, double *sc);
+void or_get_scalar_prim(const or_solver *s, double *r); /* mixing ratios r = s / rho */
 
 /* geometry tables, [p][k][j] of nc3*nc2 (cell) values; which: 0 cos, 1 sin,
  * 2 cos_face2, 3 sin_face2, 4 cos_face3, 5 sin_face3, 6 dx2f_ang, 7 dx3f_ang,
@@ -82,6 +85,11 @@ double or_cell_volume(const or_solver *s, int p, int k, int j, int i);
 double or_face_area(const or_solver *s, int p, int dir, int k, int j, int i);
 /* lat/lon of a cell center (radians) */
 void or_latlon(const or_solver *s, int p, int k, int j, double *lat, double *lon);
+/* user outputs of examples/2023-Chen-exo3/W92.cpp:115-141: lat, lon (degrees),
+ * U, V, sqrtg at one point, and at every cell: out (6, 5, nc3, nc2, nc1) */
+void or_w92_user_out_point(double x2v, double x3v, int blockID, double Vy, double Vz,
+                           double *out5);
+void or_user_output(const or_solver *s, double *out);
 
 /* pipeline pieces */
 void or_fill_ghosts(or_solver *s);          /* A7 panel remap on u (and s) */

@@ -52,6 +52,7 @@ def lib():
             "or_set_cons": (None, [V, _DP]),
             "or_get_cons": (None, [V, _DP]),
             "or_get_prim": (None, [V, _DP]),
+            "or_get_scalar_prim": (None, [V, _DP]),
             "or_set_scalars": (None, [V, _DP]),
             "or_get_scalars": (None, [V, _DP]),
             "or_get_geom2d": (None, [V, C.c_int, _DP]),
@@ -60,6 +61,9 @@ def lib():
             "or_face_area": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
             "or_latlon": (None, [V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
+            "or_w92_user_out_point": (None, [C.c_double, C.c_double, C.c_int, C.c_double,
+                                             C.c_double, _DP]),
+            "or_user_output": (None, [V, _DP]),
             "or_fill_ghosts": (None, [V]),
             "or_cons_to_prim": (None, [V]),
             "or_calc_fluxes": (None, [V]),
@@ -133,6 +137,17 @@ class Oracle:
         w = np.zeros(self.shape)
         self.L.or_get_prim(self.h, w)
         return w
+
+    def scalar_prim(self):
+        r = np.zeros(self.sshape)
+        self.L.or_get_scalar_prim(self.h, r)
+        return r
+
+    def user_output(self):
+        """lat, lon, U, V, sqrtg at every cell, (6, 5, nc3, nc2, nc1)"""
+        out = np.zeros((6, 5, self.nc2, self.nc2, self.nc1))
+        self.L.or_user_output(self.h, out)
+        return out
 
     def calc_fluxes(self):
         self.L.or_calc_fluxes(self.h)

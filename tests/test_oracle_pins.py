@@ -339,3 +339,76 @@ def test_hllc_keeps_horizontally_uniform_atmosphere(orc, ex):
     got, _ = o.download()
     gi = interior(got, 8)
     assert np.abs(gi[:, 2:4]).max() < 1e-11
+
+
+# --- W92 user outputs (examples/2023-Chen-exo3/W92.cpp:115-141) ---------------
+def _uov(orc, a, b, blk, vy=0.0, vz=0.0):
+    o = np.zeros(5)
+    orc.lib().or_w92_user_out_point(a, b, blk, vy, vz, o)
+    return o
+
+
+def test_w92_user_output_known_answers(orc):
+    # panel centres: the poles on blocks 1 and 5, the equator at the four
+    # longitudes of blocks 2, 3, 4, 6 (after lon -> 2 pi - lon); at an
+    # equatorial centre the basis is orthonormal, U = -v3 and V = -v2
+    centres = {1: (90.0, 360.0), 2: (0.0, 360.0), 3: (0.0, 90.0), 4: (0.0, 270.0),
+               5: (-90.0, 180.0), 6: (0.0, 180.0)}
+    for blk, (lat, lon) in centres.items():
+        o = _uov(orc, 0.0, 0.0, blk, 1.0, 2.0)
+        assert o[0] == pytest.approx(lat, abs=1e-12) and o[1] == pytest.approx(lon, abs=1e-12)
+        assert o[4] == 1.0
+        if blk not in (1, 5):
+            assert (o[2], o[3]) == (-2.0, -1.0)
+    # off the centre of block 2: x2 runs south, x3 runs west
+    o = _uov(orc, 0.1, 0.0, 2)
+    assert o[0] == pytest.approx(-math.degrees(0.1), rel=1e-14) and o[1] == 360.0
+    o = _uov(orc, 0.0, 0.1, 2)
+    assert o[0] == pytest.approx(0.0, abs=1e-15)
+    assert o[1] == pytest.approx(360.0 - math.degrees(0.1), rel=1e-14)
+    # sqrtg = delta / (C D) at tangents with rational roots:
+    # tan = 3/4, 0 -> 64/125; tan = 3/4, 4/3 -> 1728/12025
+    for ta, tb, want in ((0.75, 0.0, Fraction(64, 125)), (0.75, 4 / 3, Fraction(1728, 12025))):
+        for blk in range(1, 7):
+            assert _uov(orc, math.atan(ta), math.atan(tb), blk)[4] == \
+                pytest.approx(float(want), rel=1e-14)
+
+
+def _unit(lat, lon):
+    la, lo = math.radians(lat), math.radians(lon)
+    return np.array([math.cos(la) * math.cos(lo), math.cos(la) * math.sin(lo), math.sin(la)])
+
+
+def test_w92_user_output_wind_geometry(orc):
+    """The lat-lon positions are a rigid map of the gnomonic panel (angles
+    between points agree), the wind keeps the speed of the non-orthogonal
+    (v2, v3) (|U, V|^2 = v2^2 + v3^2 + 2 v2 v3 cos theta, cos theta =
+    -xy / (C D)) and a velocity along a coordinate line points along that
+    line's lat-lon track."""
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    for blk in range(1, 7):
+        for _ in range(40):
+            a, b = rng.uniform(-0.75, 0.75, 2)
+            a2, b2 = rng.uniform(-0.75, 0.75, 2)
+            g1 = np.array([1.0, math.tan(a), math.tan(b)])
+            g2 = np.array([1.0, math.tan(a2), math.tan(b2)])
+            cos_g = g1 @ g2 / np.linalg.norm(g1) / np.linalg.norm(g2)
+            o1, o2 = _uov(orc, a, b, blk), _uov(orc, a2, b2, blk)
+            assert _unit(o1[0], o1[1]) @ _unit(o2[0], o2[1]) == pytest.approx(cos_g, abs=1e-13)
+            vy, vz = rng.normal(size=2)
+            o = _uov(orc, a, b, blk, vy, vz)
+            x, y = math.tan(a), math.tan(b)
+            cth = -x * y / math.sqrt((1 + x * x) * (1 + y * y))
+            assert o[2] ** 2 + o[3] ** 2 == pytest.approx(vy * vy + vz * vz + 2 * vy * vz * cth,
+                                                          rel=1e-12)
+            if abs(o1[0]) > 80.0:   # the lat-lon track degenerates near a pole
+                continue
+            for e, (da, db) in enumerate(((h, 0.0), (0.0, h))):
+                p0, p1 = _uov(orc, a - da, b - db, blk), _uov(orc, a + da, b + db, blk)
+                dlon = (p1[1] - p0[1] + 180.0) % 360.0 - 180.0
+                east = math.radians(dlon) * math.cos(math.radians(o1[0]))
+                north = math.radians(p1[0] - p0[0])
+                w = _uov(orc, a, b, blk, 1.0 if e == 0 else 0.0, 1.0 if e == 1 else 0.0)
+                t = np.array([east, north]) / math.hypot(east, north)
+                assert w[2:4] / np.linalg.norm(w[2:4]) == pytest.approx(t, abs=1e-6)
