@@ -340,6 +340,54 @@ def output_bench(ex, reps=10):
     return out
 
 
+def recovery_bench(ex, steps=10):
+    """Failure recovery on the headline workload (mixed form): the in-HBM
+    snapshot a checked step takes, checked stepping against plain graph
+    replay, and a checkpoint written and read back."""
+    import tempfile
+
+    cfg = workload_config(ex, ex.PREC_MIXED)
+    u0, _ = ex.init_state(cfg, 1, SEED, IC_PARAMS)
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    del u0
+    g.step(2)
+    g.save_states()
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.save_states()
+    g.sync()
+    snap_ms = (time.perf_counter() - t0) * 1e3 / steps
+    snap_bytes = 2 * 2 * 5 * 8 * 6 * 102 * 102 * 70   # u and w with ghosts, read + write
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    plain = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    _, redos = g.step_redo(steps, 3)
+    checked = (time.perf_counter() - t0) * 1e3 / steps
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "run.ckpt")
+        t0 = time.perf_counter()
+        g.save_checkpoint(path, 0.0, 0)
+        t1 = time.perf_counter()
+        size = os.path.getsize(path)
+        h = ex.Solver(cfg)
+        t2 = time.perf_counter()
+        h.load_checkpoint(path)
+        t3 = time.perf_counter()
+        h.close()
+    g.close()
+    return {"snapshot_ms": snap_ms, "snapshot_GB_s": snap_bytes / (snap_ms * 1e-3) / 1e9,
+            "ms_per_step_graph": plain, "ms_per_step_checked": checked, "redos": redos,
+            "checkpoint_save_ms": (t1 - t0) * 1e3, "checkpoint_load_ms": (t3 - t2) * 1e3,
+            "checkpoint_MB": size / 1e6,
+            "workload": "6x96x96x64 mixed form; checked steps take a snapshot and a "
+                        "validity read per step (max_redo 3)"}
+
+
 def serving(ex, precision, nreq, keep=2):
     """Poisson request stream through the continuous-batching server."""
     rng = np.random.default_rng(SEED + 3)
@@ -459,6 +507,7 @@ def main():
                            "levels, WENO5+LMARS, implicit 5x5, 200 steps each"}
     big = large(ex, args.large_steps) if args.large_steps > 0 and not args.only_headline else None
     outp = None if args.only_headline else output_bench(ex)
+    recov = None if args.only_headline else recovery_bench(ex)
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
@@ -490,6 +539,7 @@ def main():
         "serving": srv,
         "large": big,
         "output": outp,
+        "recovery": recov,
     }
     print(json.dumps(out))
 

@@ -116,6 +116,16 @@ def lib():
         "exo_server_stats": (C.c_int, [H, C.c_void_p, C.c_int]),
         "exo_server_last_error": (C.c_char_p, [H]),
         "exo_server_destroy": (C.c_int, [H]),
+        "exo_state_valid": (C.c_int, [H, C.POINTER(C.c_int)]),
+        "exo_save_states": (C.c_int, [H]),
+        "exo_load_states": (C.c_int, [H]),
+        "exo_step_redo": (C.c_int, [H, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]),
+        "exo_get_dt_state": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "exo_set_dt_state": (C.c_int, [H, C.c_double, C.c_double]),
+        "exo_save_checkpoint": (C.c_int, [H, C.c_char_p, C.c_double, C.c_int64]),
+        "exo_load_checkpoint": (C.c_int, [H, C.c_char_p, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_int64)]),
         "exo_output_names": (C.c_int, [C.POINTER(Config), C.c_char_p, C.c_char_p, C.c_size_t,
                                        C.POINTER(C.c_int)]),
         "exo_output_fields": (C.c_int, [H, C.c_char_p, C.c_int, C.c_void_p, C.c_size_t]),
@@ -315,6 +325,45 @@ class Solver:
 
     def sync(self):
         self._chk(self.L.exo_sync(self.h), "exo_sync")
+
+    def state_valid(self):
+        v = C.c_int()
+        self._chk(self.L.exo_state_valid(self.h, C.byref(v)), "exo_state_valid")
+        return bool(v.value)
+
+    def save_states(self):
+        self._chk(self.L.exo_save_states(self.h), "exo_save_states")
+
+    def load_states(self):
+        self._chk(self.L.exo_load_states(self.h), "exo_load_states")
+
+    def step_redo(self, nsteps=1, max_redo=0):
+        """nsteps accepted steps, an invalid one undone and repeated with half
+        the time step; returns (last dt, steps redone)"""
+        d, r = C.c_double(), C.c_int()
+        self._chk(self.L.exo_step_redo(self.h, nsteps, max_redo, C.byref(d), C.byref(r)),
+                  "exo_step_redo")
+        return d.value, r.value
+
+    def dt_state(self):
+        """(dt of the last step, previous dt bounding the next one's growth)"""
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.L.exo_get_dt_state(self.h, C.byref(a), C.byref(b)), "exo_get_dt_state")
+        return a.value, b.value
+
+    def set_dt_state(self, dt, dt_prev):
+        self._chk(self.L.exo_set_dt_state(self.h, dt, dt_prev), "exo_set_dt_state")
+
+    def save_checkpoint(self, path, time=0.0, ncycle=0):
+        self._chk(self.L.exo_save_checkpoint(self.h, os.fsencode(path), time, ncycle),
+                  "exo_save_checkpoint")
+
+    def load_checkpoint(self, path):
+        """restores a checkpoint; returns (time, ncycle)"""
+        t, n = C.c_double(), C.c_int64()
+        self._chk(self.L.exo_load_checkpoint(self.h, os.fsencode(path), C.byref(t), C.byref(n)),
+                  "exo_load_checkpoint")
+        return t.value, n.value
 
     def output_fields(self, variable="prim", ghost_zones=False):
         """Float fields of an output variable, [6][nvar][x1][x2][x3] (x3 fastest)."""

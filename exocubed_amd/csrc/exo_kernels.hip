@@ -108,6 +108,8 @@ struct StateT {
                                         // the reconstructed variable's reference,
                                         // [2nc1,3nc1) rho_ref
   PTR(double) part;                     // per-block compensated partial sums [block][5][2]
+  PTR(int) bad;                         // set by the primitive conversion of the stage
+                                        // when a density or pressure is negative or NaN
 };
 
 using Grid = GridT<0>;
@@ -147,6 +149,7 @@ __device__ __forceinline__ DState to_global(const State &h) {
   }
   s.edge = glb(h.edge); s.coef = glb(h.coef); s.dtbits = glb(h.dtbits); s.dtv = glb(h.dtv);
   s.fedge = glb(h.fedge); s.psf = glb(h.psf); s.lref = glb(h.lref); s.part = glb(h.part);
+  s.bad = glb(h.bad);
   return s;
 }
 
@@ -259,6 +262,9 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
     wv[IVY] = w_vy;
     wv[IVZ] = w_vz;
     wv[IPR] = gm1 * (u_e - KE) * 1. / 1.;
+    // ideal_moist_hydro.cpp:108,150: a negative or NaN density or pressure
+    // marks the state invalid (the shallow-water EOS has no such check)
+    if (!(density >= 0.) || !(wv[IPR] >= 0.)) *st.bad = 1;
   }
   // mirrors into ghosts is - qb, ie + qt (shallow water never reads x1 ghosts)
   int qb = i - g.is + 1, qt = g.ie - i + 1;
@@ -926,6 +932,10 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
 template <bool MIX>
 __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
+  // the validity flag describes the primitive conversion that follows this
+  // update (EquationOfState::ConservedToPrimitive resets it on entry,
+  // ideal_moist_hydro.cpp:83)
+  if (t == 0) *st.bad = 0;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
   if (t >= total) return;
@@ -2002,6 +2012,8 @@ struct exo_handle {
   double *d_uov = nullptr;      // per-column factors of the "uov" output (first use)
   float *d_out = nullptr;       // output fields (grown on demand)
   size_t out_cap = 0;           // floats in d_out
+  double *d_snap = nullptr;     // Mesh::SaveAllStates snapshot: u, w, s, r, dtv (first use)
+  int redo_attempts = 0;        // redone steps over the handle's life (main.cpp.in:127)
   bool have_state = false;
   bool mixed = false;
   // one SSPRK3 step captured as a HIP graph and replayed by exo_step
@@ -2290,6 +2302,7 @@ void init_handle(exo_handle *h) {
   if (h->mixed) st.lref = (double *)h->dalloc((size_t)3 * g.nc1 * sizeof(double));
   st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
   st.dtv = (double *)h->dalloc(2 * sizeof(double));
+  st.bad = (int *)h->dalloc(sizeof(int));
   st.part = (double *)h->dalloc((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
   h->d_sums = (double *)h->dalloc(5 * sizeof(double));
   double dt0[2] = {0., DBL_MAX * 0.4};
@@ -2681,6 +2694,7 @@ int exo_upload_state(exo_handle *h, const double *u, const double *s) {
     double dt0[2] = {0., DBL_MAX * 0.4};
     HIPCHK(hipMemcpyAsync(h->st.dtv, dt0, sizeof(dt0), hipMemcpyHostToDevice, h->stream));
     if (h->mixed) set_reference(h, u);
+    HIPCHK(hipMemsetAsync(h->st.bad, 0, sizeof(int), h->stream));
     launch_fill_ghosts(h->L);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_state = true;
@@ -2800,6 +2814,74 @@ void handle_graph(exo_handle *h, double dt_fixed) {
   HIPCHK(hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0));
   h->graph_dt_fixed = dt_fixed;
 }
+
+// Mesh::SaveAllStates / LoadAllStates (src/mesh_redo.cpp:12-24 over
+// MeshBlock::Impl::SaveAllStates, impl.cpp:306-332): the conserved and
+// primitive state, here with the passive scalars and the time-step state,
+// copied within HBM
+size_t snap_doubles(const exo_handle *h) {
+  return (size_t)(2 * h->g.nv + 2 * h->g.nt) * h->g.vstride + 2;
+}
+
+void snap_copy(exo_handle *h, bool save) {
+  const Grid &g = h->g;
+  double *d = h->d_snap;
+  auto cp = [&](double *state, size_t n) {
+    HIPCHK(hipMemcpyAsync(save ? d : state, save ? state : d, n * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->stream));
+    d += n;
+  };
+  cp(h->st.u, (size_t)g.nv * g.vstride);
+  cp(h->st.w, (size_t)g.nv * g.vstride);
+  if (g.nt) {
+    cp(h->st.s, (size_t)g.nt * g.vstride);
+    cp(h->st.r, (size_t)g.nt * g.vstride);
+  }
+  cp(h->st.dtv, 2);
+}
+
+void save_states(exo_handle *h) {
+  if (!h->d_snap) h->d_snap = (double *)h->dalloc(snap_doubles(h) * sizeof(double));
+  snap_copy(h, true);
+}
+
+void load_states(exo_handle *h) {
+  if (!h->d_snap) throw std::string("no saved state");
+  snap_copy(h, false);
+}
+
+bool state_valid(exo_handle *h) {
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, h->st.bad, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return bad == 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint files: everything a restart needs to continue bit for bit
+constexpr char CKPT_MAGIC[8] = {'E', 'X', 'O', 'C', 'K', 'P', 'T', '1'};
+
+struct CkptMeta {
+  double time, dt, dt_prev;
+  int64_t ncycle;
+};
+
+void put_cfg(std::vector<double> &m, const exo_config &c) {
+  for (int v : {c.nx, c.nx1, c.eos, c.rsolver, c.implicit_flag, c.ntracer, c.coriolis,
+                c.cfl_x1, c.precision})
+    m.push_back((double)v);
+  for (double v : {c.x1min, c.x1max, c.gamma, c.grav, c.omega, c.cfl}) m.push_back(v);
+}
+
+// FNV-1a over the payload's 64-bit words
+uint64_t fnv64(const double *a, size_t n, uint64_t hsh) {
+  for (size_t e = 0; e < n; ++e) {
+    uint64_t x;
+    std::memcpy(&x, a + e, 8);
+    hsh = (hsh ^ x) * 0x100000001b3ull;
+  }
+  return hsh;
+}
 }  // namespace
 
 int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
@@ -2817,6 +2899,200 @@ int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
       HIPCHK(hipStreamSynchronize(h->stream));
       *dt_last = v[0];
     }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// redo / rollback (main.cpp.in:126-160, src/mesh_redo.cpp:12-45)
+int exo_state_valid(exo_handle *h, int *valid) {
+  return guarded(h, [&] {
+    if (!valid) throw std::string("null output");
+    *valid = state_valid(h) ? 1 : 0;
+  });
+}
+
+int exo_save_states(exo_handle *h) {
+  return guarded(h, [&] {
+    if (!h->have_state) throw std::string("no state uploaded");
+    save_states(h);
+  });
+}
+
+int exo_load_states(exo_handle *h) {
+  return guarded(h, [&] {
+    load_states(h);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_step_redo(exo_handle *h, int nsteps, int max_redo, double *dt_last, int *redos) {
+  return guarded(h, [&] {
+    if (!h->have_state) throw std::string("no state uploaded");
+    bool redo = false;
+    double dt_redo = 0.;
+    int done = 0, n_redo = 0;
+    if (redos) *redos = 0;
+    while (done < nsteps) {
+      if (!redo && max_redo > 0) save_states(h);
+      if (redo) {
+        // Mesh::DecreaseTimeStep: the halved dt is the step's dt, and the
+        // next step may grow to twice it (Mesh::NewTimeStep)
+        double v[2] = {dt_redo, dt_redo};
+        HIPCHK(hipMemcpyAsync(h->st.dtv, v, sizeof(v), hipMemcpyHostToDevice, h->stream));
+        run_step(h->L, dt_redo);
+      } else {
+        run_step(h->L, 0.0);
+      }
+      if (max_redo > 0) {
+        if (state_valid(h)) {
+          redo = false;
+        } else {
+          double v[2];
+          HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+          HIPCHK(hipStreamSynchronize(h->stream));
+          load_states(h);
+          dt_redo = v[0] / 2;
+          redo = true;
+          ++n_redo;
+          if (redos) *redos = n_redo;
+          if (h->redo_attempts++ > max_redo)
+            throw std::string("Too many attempts to redo the step");
+          continue;
+        }
+      }
+      ++done;
+    }
+    if (dt_last) {
+      double v[2];
+      HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+      HIPCHK(hipStreamSynchronize(h->stream));
+      *dt_last = v[0];
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+int exo_get_dt_state(exo_handle *h, double *dt, double *dt_prev) {
+  return guarded(h, [&] {
+    double v[2];
+    HIPCHK(hipMemcpyAsync(v, h->st.dtv, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (dt) *dt = v[0];
+    if (dt_prev) *dt_prev = v[1];
+  });
+}
+
+int exo_set_dt_state(exo_handle *h, double dt, double dt_prev) {
+  return guarded(h, [&] {
+    if (!(dt_prev > 0.)) throw std::string("dt_prev must be positive");
+    double v[2] = {dt, dt_prev};
+    HIPCHK(hipMemcpyAsync(h->st.dtv, v, sizeof(v), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint / restart
+int exo_save_checkpoint(exo_handle *h, const char *path, double time, int64_t ncycle) {
+  return guarded(h, [&] {
+    if (!path) throw std::string("null path");
+    if (!h->have_state) throw std::string("no state uploaded");
+    const Grid &g = h->g;
+    size_t nu = (size_t)6 * g.nv * g.nc3 * g.nc2 * g.nc1;
+    size_t ns = (size_t)6 * g.nt * g.nc3 * g.nc2 * g.nc1;
+    size_t nl = h->mixed ? 3 * (size_t)g.nc1 : 0;
+    std::vector<double> pay(nu + ns + nl);
+    dev_to_host(h, h->st.u, pay.data(), g.nv);
+    if (g.nt) dev_to_host(h, h->st.s, pay.data() + nu, g.nt);
+    if (nl)
+      HIPCHK(hipMemcpyAsync(pay.data() + nu + ns, h->st.lref, nl * sizeof(double),
+                            hipMemcpyDeviceToHost, h->stream));
+    double dtv[2];
+    HIPCHK(hipMemcpyAsync(dtv, h->st.dtv, sizeof(dtv), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> meta;
+    put_cfg(meta, h->setup.cfg);
+    meta.push_back(time);
+    meta.push_back(dtv[0]);
+    meta.push_back(dtv[1]);
+    double nc;
+    std::memcpy(&nc, &ncycle, 8);  // the cycle count's bits
+    meta.push_back(nc);
+    uint64_t counts[3] = {nu, ns, nl};
+    uint64_t sum = fnv64(pay.data(), pay.size(), fnv64(meta.data(), meta.size(),
+                                                        0xcbf29ce484222325ull));
+    std::string tmp = std::string(path) + ".tmp";
+    std::FILE *fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) throw std::string("cannot open ") + tmp + " for writing";
+    uint64_t nmeta = meta.size();
+    bool ok = std::fwrite(CKPT_MAGIC, 1, 8, fp) == 8 &&
+              std::fwrite(&nmeta, 8, 1, fp) == 1 &&
+              std::fwrite(meta.data(), 8, nmeta, fp) == nmeta &&
+              std::fwrite(counts, 8, 3, fp) == 3 &&
+              std::fwrite(pay.data(), 8, pay.size(), fp) == pay.size() &&
+              std::fwrite(&sum, 8, 1, fp) == 1;
+    if (std::fclose(fp) != 0) ok = false;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) {
+      std::remove(tmp.c_str());
+      throw std::string("write error on ") + path;
+    }
+  });
+}
+
+int exo_load_checkpoint(exo_handle *h, const char *path, double *time, int64_t *ncycle) {
+  return guarded(h, [&] {
+    if (!path) throw std::string("null path");
+    std::FILE *fp = std::fopen(path, "rb");
+    if (!fp) throw std::string("cannot open ") + path;
+    std::vector<double> meta(19), pay;
+    uint64_t nmeta = 0, counts[3] = {0, 0, 0}, sum = 0;
+    char magic[8];
+    bool ok = std::fread(magic, 1, 8, fp) == 8 && !std::memcmp(magic, CKPT_MAGIC, 8) &&
+              std::fread(&nmeta, 8, 1, fp) == 1 && nmeta == meta.size() &&
+              std::fread(meta.data(), 8, nmeta, fp) == nmeta && std::fread(counts, 8, 3, fp) == 3;
+    if (!ok) {
+      std::fclose(fp);
+      throw std::string("not a checkpoint file: ") + path;
+    }
+    const Grid &g = h->g;
+    size_t nu = (size_t)6 * g.nv * g.nc3 * g.nc2 * g.nc1;
+    size_t ns = (size_t)6 * g.nt * g.nc3 * g.nc2 * g.nc1;
+    std::vector<double> mine;
+    put_cfg(mine, h->setup.cfg);
+    // the configuration must match except for the precision
+    for (int q = 0; q < 15; ++q)
+      if (q != 8 && std::memcmp(&mine[q], &meta[q], 8) != 0) {
+        std::fclose(fp);
+        throw std::string("checkpoint of another configuration: ") + path;
+      }
+    ok = counts[0] == nu && counts[1] == ns && (counts[2] == 0 || counts[2] == 3 * (size_t)g.nc1);
+    if (ok) {
+      pay.resize(counts[0] + counts[1] + counts[2]);
+      ok = std::fread(pay.data(), 8, pay.size(), fp) == pay.size() &&
+           std::fread(&sum, 8, 1, fp) == 1;
+    }
+    std::fclose(fp);
+    if (!ok) throw std::string("not a complete checkpoint file: ") + path;
+    if (fnv64(pay.data(), pay.size(), fnv64(meta.data(), meta.size(), 0xcbf29ce484222325ull)) !=
+        sum)
+      throw std::string("checkpoint checksum mismatch: ") + path;
+    host_to_dev(h, pay.data(), h->st.u, g.nv);
+    if (g.nt) host_to_dev(h, pay.data() + nu, h->st.s, g.nt);
+    double dtv[2] = {meta[16], meta[17]};
+    HIPCHK(hipMemcpyAsync(h->st.dtv, dtv, sizeof(dtv), hipMemcpyHostToDevice, h->stream));
+    if (h->mixed) {
+      if (counts[2])  // the reference levels of the run that wrote it
+        HIPCHK(hipMemcpyAsync(h->st.lref, pay.data() + nu + ns, counts[2] * sizeof(double),
+                              hipMemcpyHostToDevice, h->stream));
+      else
+        set_reference(h, pay.data());
+    }
+    HIPCHK(hipMemsetAsync(h->st.bad, 0, sizeof(int), h->stream));
+    launch_fill_ghosts(h->L);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_state = true;
+    if (time) *time = meta[15];
+    if (ncycle) std::memcpy(ncycle, &meta[18], 8);
   });
 }
 

@@ -280,6 +280,57 @@ EXO_API int exo_write_netcdf_fields(const exo_config *cfg, const char *basename,
                                     double time, int blocks_per_side, int ghost_zones,
                                     const float *fields);
 
+/* --- failure recovery: redo a step with half the time step ---------------- */
+/* The reference's main loop (main.cpp.in:126-160) keeps a copy of the state
+ * before each step when max_redo > 0 (Mesh::SaveAllStates,
+ * src/mesh_redo.cpp:12-17), and after the step checks that every block's
+ * primitive conversion found no negative or NaN density or pressure
+ * (ideal_moist_hydro.cpp:83,108,150; Mesh::CheckAllValid, mesh_redo.cpp:31-45).
+ * An invalid step is undone (LoadAllStates) and repeated with half the time
+ * step (DecreaseTimeStep); more than max_redo + 1 redone steps over the run
+ * fail with "Too many attempts to redo the step".
+ *   exo_state_valid: 1 if the last stage's primitive conversion (or the
+ *     upload's) found every density and pressure non-negative, else 0; the
+ *     shallow-water EOS never marks a state invalid.
+ *   exo_save_states / exo_load_states: snapshot of the conserved, primitive
+ *     and scalar state and the time-step state within HBM, and its restore.
+ *   exo_step_redo: nsteps accepted SSPRK3 steps (CFL time step) with that
+ *     redo rule; *redos = steps redone by this call.  max_redo = 0 steps as
+ *     exo_step does, without checks (the reference's default). */
+EXO_API int exo_state_valid(exo_handle *h, int *valid);
+EXO_API int exo_save_states(exo_handle *h);
+EXO_API int exo_load_states(exo_handle *h);
+EXO_API int exo_step_redo(exo_handle *h, int nsteps, int max_redo, double *dt_last, int *redos);
+
+/* --- time-step state ------------------------------------------------------ */
+/* dt of the last step and the previous dt that bounds the next CFL time step
+ * to twice it (Mesh::NewTimeStep, upstream Athena; patches/21.new_blockdt.patch
+ * computes the block limit).  exo_upload_state resets it to (0, unbounded).  A
+ * host that restarts from its own restart file (which stores Mesh::dt) sets
+ * dt_prev to that dt so the run continues with the same time steps. */
+EXO_API int exo_get_dt_state(exo_handle *h, double *dt, double *dt_prev);
+EXO_API int exo_set_dt_state(exo_handle *h, double dt, double dt_prev);
+
+/* --- checkpoint / restart ------------------------------------------------- */
+/* Replaces the restart round trip of the reference (RestartOutput writing
+ * <basename>.rst, and Mesh(pinput, restartfile) reading it in mesh_setup,
+ * src/mesh_setup.cpp:31-39,64-69) for this instance: the conserved state
+ * with its ghost cells, the passive scalars, the time-step state (dt and the
+ * previous dt that bounds its growth), the mixed form's per-level reference,
+ * and the caller's time and cycle count.  A run restarted from a checkpoint
+ * continues bit for bit as the uninterrupted run.  The file (little-endian) is
+ * "EXOCKPT1", a count of metadata doubles (19: the exo_config fields, time,
+ * dt, previous dt, cycle-count bits), the payload's three double counts, the
+ * payload (u, s and the reference levels in the host layout of
+ * exo_upload_state) and an FNV-1a checksum of metadata and payload words.  It
+ * is written to <path>.tmp and renamed, so an interrupted save leaves any
+ * earlier checkpoint at <path> intact.  Loading requires the same
+ * configuration except for the precision (a checkpoint of one precision
+ * restarts in the other; an FP64 checkpoint gives a mixed handle the
+ * reference levels of the restored state). */
+EXO_API int exo_save_checkpoint(exo_handle *h, const char *path, double time, int64_t ncycle);
+EXO_API int exo_load_checkpoint(exo_handle *h, const char *path, double *time, int64_t *ncycle);
+
 #ifdef __cplusplus
 }
 #endif

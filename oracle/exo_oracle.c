@@ -57,6 +57,9 @@ struct or_solver {
   int is, ie, js, je, ks, ke;
   double dfloor, sfloor;
   double dt_prev;
+  int valid;      /* MeshBlock::Impl::state_valid_flag_ (impl.hpp:114-121) */
+  int attempts;   /* redone steps over the run (main.cpp.in:127) */
+  double *save;   /* MeshBlock::Impl::mbdata_ (impl.cpp:80-86): u, w, s, r, dt_prev */
   or_panel pb[6];
 };
 
@@ -665,6 +668,11 @@ static void cons2prim_cell(or_solver *s, or_panel *P, int k, int j, int i) {
     P->w[C4(IVY, k, j, i)] = w_vy;
     P->w[C4(IVZ, k, j, i)] = w_vz;
     P->w[C4(IPR, k, j, i)] = gm1 * (u_e - KE) * feps / fsig;
+    /* ideal_moist_hydro.cpp:108,150 */
+    if (!(density >= 0.) || !(P->w[C4(IPR, k, j, i)] >= 0.)) {
+#pragma omp atomic write
+      s->valid = 0;
+    }
   }
   for (int n = 0; n < s->nt; ++n) {
     double r = P->s[C4(n, k, j, i)] * (1.0 / P->u[C4(IDN, k, j, i)]);
@@ -688,6 +696,7 @@ static void reflect_x1(or_solver *s, or_panel *P, int k, int j) {
 }
 
 void or_cons_to_prim(or_solver *s) {
+  s->valid = 1; /* ideal_moist_hydro.cpp:83 */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int p = 0; p < 6; ++p)
     for (int k = 0; k < s->nc3; ++k) {
@@ -1722,6 +1731,71 @@ double or_step(or_solver *s, int nsteps, double dt_fixed) {
   return dt;
 }
 
+/* Mesh::SaveAllStates / LoadAllStates (src/mesh_redo.cpp:12-24,
+ * impl.cpp:306-332): u and w, here with the scalars and dt_prev */
+static size_t save_doubles(const or_solver *s) {
+  return (size_t)6 * (2 * s->nv + 2 * s->nt) * s->nc3 * s->nc2 * s->nc1 + 1;
+}
+
+static void save_load(or_solver *s, int save) {
+  if (!s->save) s->save = calloc(save_doubles(s), sizeof(double));
+  double *d = s->save;
+  size_t nvv = (size_t)s->nv * s->nc3 * s->nc2 * s->nc1;
+  size_t ntt = (size_t)s->nt * s->nc3 * s->nc2 * s->nc1;
+  for (int p = 0; p < 6; ++p) {
+    or_panel *P = &s->pb[p];
+    double *arr[4] = {P->u, P->w, P->s, P->r};
+    size_t n[4] = {nvv, nvv, ntt, ntt};
+    for (int a = 0; a < 4; ++a) {
+      if (!n[a]) continue;
+      if (save) memcpy(d, arr[a], n[a] * sizeof(double));
+      else memcpy(arr[a], d, n[a] * sizeof(double));
+      d += n[a];
+    }
+  }
+  if (save) *d = s->dt_prev;
+  else s->dt_prev = *d;
+}
+
+/* The main loop's redo path, main.cpp.in:126-160: with max_redo > 0 the state
+ * is saved before a step; if the last primitive conversion of the step marked
+ * it invalid, it is restored, dt halved (Mesh::DecreaseTimeStep,
+ * mesh_redo.cpp:26-29) and the step repeated; more than max_redo + 1 redone
+ * steps over the run fail (returns -1).  Returns the number of steps redone. */
+int or_step_redo(or_solver *s, int nsteps, int max_redo, double *dt_last) {
+  int done = 0, redo = 0, nredo = 0;
+  double dt = 0.;
+  while (done < nsteps) {
+    if (!redo && max_redo > 0) save_load(s, 1);
+    if (redo) {
+      s->dt_prev = dt; /* Mesh::NewTimeStep grows from the halved dt */
+    } else {
+      double d = or_new_dt(s);
+      dt = 2.0 * s->dt_prev;
+      if (d < dt) dt = d;
+      s->dt_prev = dt;
+    }
+    for (int st = 1; st <= 3; ++st) or_stage(s, st, dt);
+    if (max_redo > 0) {
+      if (s->valid) {
+        redo = 0;
+      } else {
+        save_load(s, 0);
+        dt /= 2;
+        redo = 1;
+        ++nredo;
+        if (s->attempts++ > max_redo) return -1;
+        continue;
+      }
+    }
+    ++done;
+  }
+  if (dt_last) *dt_last = dt;
+  return nredo;
+}
+
+int or_state_valid(const or_solver *s) { return s->valid; }
+
 /* ------------------------------------------------------------------------ */
 or_solver *or_create(const or_config *cfg) {
   or_solver *s = calloc(1, sizeof(or_solver));
@@ -1784,6 +1858,7 @@ void or_destroy(or_solver *s) {
                       P->sflx[0], P->sflx[1], P->sflx[2], P->L[1], P->L[2], P->R[1], P->R[2]};
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
   }
+  free(s->save);
   free(s);
 }
 

@@ -99,6 +99,11 @@ void or_get_flux(const or_solver *s, int dir, double *out); /* [p][n][..] */
 double or_new_dt(or_solver *s);             /* A14 */
 void or_stage(or_solver *s, int stage, double dt); /* one SSPRK3 stage */
 double or_step(or_solver *s, int nsteps, double dt_fixed); /* returns last dt */
+/* the main loop's redo path (main.cpp.in:126-160): nsteps accepted steps,
+ * an invalid step undone and repeated with half the dt; returns the steps
+ * redone, -1 after too many */
+int or_step_redo(or_solver *s, int nsteps, int max_redo, double *dt_last);
+int or_state_valid(const or_solver *s); /* last primitive conversion valid */
 void or_set_threads(int n);
 
 /* single-column / single-row shims used by the per-function parity tests */

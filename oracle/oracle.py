@@ -72,6 +72,8 @@ def lib():
             "or_stage": (None, [V, C.c_int, C.c_double]),
             "or_step": (C.c_double, [V, C.c_int, C.c_double]),
             "or_set_threads": (None, [C.c_int]),
+            "or_step_redo": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+            "or_state_valid": (C.c_int, [V]),
             "or_implicit_column": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
                                           C.c_double]),
             "or_flux_jacobian": (None, [C.c_double, _DP, _DP]),
@@ -162,6 +164,16 @@ class Oracle:
 
     def step(self, nsteps=1, dt=0.0):
         return self.L.or_step(self.h, nsteps, dt)
+
+    def step_redo(self, nsteps=1, max_redo=0):
+        d = C.c_double()
+        r = self.L.or_step_redo(self.h, nsteps, max_redo, C.byref(d))
+        if r < 0:
+            raise RuntimeError("Too many attempts to redo the step")
+        return d.value, r
+
+    def state_valid(self):
+        return bool(self.L.or_state_valid(self.h))
 
     def stage(self, stage, dt):
         self.L.or_stage(self.h, stage, dt)

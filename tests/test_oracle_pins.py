@@ -412,3 +412,48 @@ def test_w92_user_output_wind_geometry(orc):
                 w = _uov(orc, a, b, blk, 1.0 if e == 0 else 0.0, 1.0 if e == 1 else 0.0)
                 t = np.array([east, north]) / math.hypot(east, north)
                 assert w[2:4] / np.linalg.norm(w[2:4]) == pytest.approx(t, abs=1e-6)
+
+
+# --- redo path (main.cpp.in:126-160) -------------------------------------------
+def test_redo_path(orc, ex):
+    """Without an invalid step the redo loop is plain stepping; with one, the
+    accepted step is a step from the restored state with the CFL dt halved once
+    per redo, and too many redos fail."""
+    cfg = atm_config(ex, nx=6, nx1=6, rsolver=3, implicit=9)
+    u0, _ = ex.init_state(cfg, 1, 20261202, ())
+    a, b = orc.Oracle(cfg), orc.Oracle(cfg)
+    a.upload(u0)
+    b.upload(u0)
+    assert a.step_redo(3, 5) == (b.step(3), 0)
+    assert np.array_equal(a.download()[0], b.download()[0])
+    cfg.cfl = 1.5
+    a, b = orc.Oracle(cfg), orc.Oracle(cfg)
+    a.upload(u0)
+    b.upload(u0)
+    dt0 = b.new_dt()
+    dt, nredo = a.step_redo(1, 5)
+    assert nredo == 2 and a.state_valid()
+    assert dt == dt0 / 4
+    b.step(1, dt)
+    assert np.array_equal(a.download()[0], b.download()[0])
+    # the failing step itself leaves an invalid state
+    c = orc.Oracle(cfg)
+    c.upload(u0)
+    c.step(1)
+    assert not c.state_valid()
+    # max_redo 0 (the reference's default) steps without checks
+    a = orc.Oracle(cfg)
+    a.upload(u0)
+    assert a.step_redo(1, 0)[1] == 0 and not a.state_valid()
+    # four explicit steps at CFL 8 need six redos; max_redo 2 allows three
+    # over the run
+    cfg = atm_config(ex, nx=6, nx1=6, rsolver=3, implicit=0)
+    cfg.cfl = 8.0
+    u0, _ = ex.init_state(cfg, 1, 20261202, ())
+    a = orc.Oracle(cfg)
+    a.upload(u0)
+    assert a.step_redo(4, 10)[1] == 6
+    a = orc.Oracle(cfg)
+    a.upload(u0)
+    with pytest.raises(RuntimeError, match="Too many"):
+        a.step_redo(4, 2)
