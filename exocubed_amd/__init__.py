@@ -297,6 +297,17 @@ class Solver:
         self._chk(self.L.exo_download_flux(self.h, d, out), "exo_download_flux")
         return out
 
+    def add_flux_divergence(self, wght):
+        """-div F + sources times wght (stage-1 weights) into u, or into the
+        tendency du for the ideal gas"""
+        self._chk(self.L.exo_add_flux_divergence(self.h, wght), "exo_add_flux_divergence")
+
+    def fill_ghosts(self):
+        self._chk(self.L.exo_fill_ghosts(self.h), "exo_fill_ghosts")
+
+    def implicit_correct(self, dt_stage):
+        self._chk(self.L.exo_implicit_correct(self.h, dt_stage), "exo_implicit_correct")
+
     def step(self, nsteps=1, dt=0.0):
         d = C.c_double()
         self._chk(self.L.exo_step(self.h, nsteps, dt, C.byref(d)), "exo_step")

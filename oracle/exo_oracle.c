@@ -1672,21 +1672,12 @@ static void weighted_ave(or_solver *s, double *u, double *u1, int nv, int st) {
         }
 }
 
-void or_stage(or_solver *s, int stage, double dt) {
-  int st = stage - 1;
-  if (st == 0) {
-    for (int p = 0; p < 6; ++p) {
-      size_t n4 = (size_t)s->nv * s->nc3 * s->nc2 * s->nc1;
-      memset(s->pb[p].u1, 0, n4 * sizeof(double));
-      if (s->nt) memset(s->pb[p].s1, 0, (size_t)s->nt * s->nc3 * s->nc2 * s->nc1 * sizeof(double));
-    }
-  }
-  or_calc_fluxes(s);
-  double wght = BETA[st] * dt;
-  /* exo3 runs ImplicitHydroTasks (tendency into du, then u += du) whether or
-   * not the column solve is enabled (cmake/exo3.cmake:15) */
+/* the SSPRK3 weighted average, then -div F + coordinate and physical sources
+ * times wght into u (shallow water) or into the zeroed tendency du (ideal gas:
+ * exo3 runs ImplicitHydroTasks whether or not the column solve is enabled,
+ * cmake/exo3.cmake:15; implicit_hydro_tasks.cpp:147-162) */
+static void stage_update(or_solver *s, int st, double wght) {
   int implicit = s->cfg.eos == OR_EOS_IDEAL;
-  int solve = implicit && (s->cfg.implicit_flag & 1) && (s->cfg.implicit_flag & 8);
   for (int p = 0; p < 6; ++p) {
     or_panel *P = &s->pb[p];
     weighted_ave(s, P->u, P->u1, s->nv, st);
@@ -1703,19 +1694,49 @@ void or_stage(or_solver *s, int stage, double dt) {
       flux_divergence(s, P, wght, P->s, P->sflx, s->nt, 0);
     }
   }
-  if (implicit) {
-    if (solve) implicit_correct(s, wght);
-    for (int p = 0; p < 6; ++p) {
-      or_panel *P = &s->pb[p];
-      for (int n = 0; n < s->nv; ++n)
-        for (int k = s->ks; k <= s->ke; ++k)
-          for (int j = s->js; j <= s->je; ++j)
-            for (int i = s->is; i <= s->ie; ++i) P->u[C4(n, k, j, i)] += P->du[C4(n, k, j, i)];
-    }
+}
+
+/* ImplicitHydroTasks::ImplicitCorrection (implicit_hydro_tasks.cpp:245-266):
+ * the column solve on du when enabled, then u += du */
+static void stage_implicit(or_solver *s, double wght) {
+  if (s->cfg.eos != OR_EOS_IDEAL) return;
+  int solve = (s->cfg.implicit_flag & 1) && (s->cfg.implicit_flag & 8);
+  if (solve) implicit_correct(s, wght);
+  for (int p = 0; p < 6; ++p) {
+    or_panel *P = &s->pb[p];
+    for (int n = 0; n < s->nv; ++n)
+      for (int k = s->ks; k <= s->ke; ++k)
+        for (int j = s->js; j <= s->je; ++j)
+          for (int i = s->is; i <= s->ie; ++i) P->u[C4(n, k, j, i)] += P->du[C4(n, k, j, i)];
   }
+}
+
+static void clear_stage_registers(or_solver *s) {
+  for (int p = 0; p < 6; ++p) {
+    size_t n4 = (size_t)s->nv * s->nc3 * s->nc2 * s->nc1;
+    memset(s->pb[p].u1, 0, n4 * sizeof(double));
+    if (s->nt) memset(s->pb[p].s1, 0, (size_t)s->nt * s->nc3 * s->nc2 * s->nc1 * sizeof(double));
+  }
+}
+
+void or_stage(or_solver *s, int stage, double dt) {
+  int st = stage - 1;
+  if (st == 0) clear_stage_registers(s);
+  or_calc_fluxes(s);
+  double wght = BETA[st] * dt;
+  stage_update(s, st, wght);
+  stage_implicit(s, wght);
   or_fill_ghosts(s);
   or_cons_to_prim(s);
 }
+
+/* the boundary's single phases (stage-1 weights, so u is kept and u1 = u) */
+void or_add_flux_divergence(or_solver *s, double wght) {
+  clear_stage_registers(s);
+  stage_update(s, 0, wght);
+}
+
+void or_implicit_correct(or_solver *s, double dt_stage) { stage_implicit(s, dt_stage); }
 
 double or_step(or_solver *s, int nsteps, double dt_fixed) {
   double dt = dt_fixed;

@@ -98,6 +98,10 @@ void or_calc_fluxes(or_solver *s);          /* A1+A2+A3+A6+A9 from w */
 void or_get_flux(const or_solver *s, int dir, double *out); /* [p][n][..] */
 double or_new_dt(or_solver *s);             /* A14 */
 void or_stage(or_solver *s, int stage, double dt); /* one SSPRK3 stage */
+/* single phases of a stage: -div F + sources times wght with stage-1 weights
+ * (into u, or into du for the ideal gas), and the implicit correction */
+void or_add_flux_divergence(or_solver *s, double wght);
+void or_implicit_correct(or_solver *s, double dt_stage);
 double or_step(or_solver *s, int nsteps, double dt_fixed); /* returns last dt */
 /* the main loop's redo path (main.cpp.in:126-160): nsteps accepted steps,
  * an invalid step undone and repeated with half the dt; returns the steps

@@ -72,6 +72,8 @@ def lib():
             "or_stage": (None, [V, C.c_int, C.c_double]),
             "or_step": (C.c_double, [V, C.c_int, C.c_double]),
             "or_set_threads": (None, [C.c_int]),
+            "or_add_flux_divergence": (None, [V, C.c_double]),
+            "or_implicit_correct": (None, [V, C.c_double]),
             "or_step_redo": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_double)]),
             "or_state_valid": (C.c_int, [V]),
             "or_implicit_column": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
@@ -164,6 +166,16 @@ class Oracle:
 
     def step(self, nsteps=1, dt=0.0):
         return self.L.or_step(self.h, nsteps, dt)
+
+    def add_flux_divergence(self, wght):
+        self.L.or_add_flux_divergence(self.h, wght)
+
+    def implicit_correct(self, dt_stage):
+        self.L.or_implicit_correct(self.h, dt_stage)
+
+    def fill_ghosts(self):
+        self.L.or_fill_ghosts(self.h)
+        self.L.or_cons_to_prim(self.h)
 
     def step_redo(self, nsteps=1, max_redo=0):
         d = C.c_double()

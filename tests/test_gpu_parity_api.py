@@ -134,3 +134,49 @@ def test_three_or_more_tracers(ex, orc, which, nt):
     assert np.array_equal(interior(sg, nx1), interior(so, nx1))
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers"])
+def test_stage_by_entry_points(ex, orc, which):
+    """One stage through the boundary's single phases (exo_calc_fluxes,
+    exo_add_flux_divergence, exo_implicit_correct, exo_fill_ghosts), each
+    against the oracle's phase, and equal to the fused exo_stage."""
+    if which == "sw":
+        cfg, kind, nx1 = sw_config(ex, nx=8, rsolver=1), 0, 3
+    else:
+        cfg, kind, nx1 = atm_config(ex, nx=6, nx1=8, rsolver=3, coriolis=2,
+                                    ntracer=2 if which == "atm_tracers" else 0), 1, 8
+    u0, _ = ex.init_state(cfg, kind, SEED, ())
+    s0 = None
+    if cfg.ntracer:
+        s0 = np.random.default_rng(4).uniform(0, 1, (6, 2) + u0.shape[2:]) * u0[:, :1]
+    g = ex.Solver(cfg)
+    g.upload(u0, s0)
+    o = orc.Oracle(cfg)
+    o.upload(u0, s0)
+    f = ex.Solver(cfg)
+    f.upload(u0, s0)
+    dt = g.new_dt()
+    g.calc_fluxes()
+    o.calc_fluxes()
+    g.add_flux_divergence(dt)
+    o.add_flux_divergence(dt)
+    if cfg.eos == ex.EOS_SHALLOW_YZ:   # the update went into u
+        assert np.array_equal(interior(g.download()[0], nx1), interior(o.download()[0], nx1))
+    g.implicit_correct(dt)
+    o.implicit_correct(dt)
+    (ug, sg), (uo, so) = g.download(), o.download()
+    assert np.array_equal(interior(ug, nx1), interior(uo, nx1))
+    if cfg.ntracer:
+        assert np.array_equal(interior(sg, nx1), interior(so, nx1))
+    g.fill_ghosts()
+    o.fill_ghosts()
+    ug, uo = g.download()[0], o.download()[0]
+    edge = (slice(None), slice(None), slice(3, -3), slice(0, 3), slice(3, 3 + nx1))
+    assert np.array_equal(ug[edge], uo[edge])          # a ghost strip
+    wg, wo = g.prim(), o.prim()
+    assert np.array_equal(interior(wg, nx1), interior(wo, nx1))
+    f.stage(1, dt)
+    assert np.array_equal(interior(f.download()[0], nx1), interior(ug, nx1))
+    for h in (g, o, f):
+        h.close()
