@@ -145,7 +145,10 @@ EXO_API int exo_conserved_sums(exo_handle *h, double *out5);
 EXO_API int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
 
 /* Device bytes exo_create allocates for this configuration (host only; 0 for
- * an invalid configuration).  Used to admit work by memory footprint. */
+ * an invalid configuration).  Used to admit work by memory footprint.  Not
+ * included: what a handle allocates on first use of exo_output_fields /
+ * exo_write_netcdf (the float fields of one output, plus the "uov" factors) and
+ * of exo_save_states / exo_step_redo (one copy of u, w and the scalars). */
 EXO_API size_t exo_footprint(const exo_config *cfg);
 
 /* --- batches of independent instances ----------------------------------- */
