@@ -97,6 +97,13 @@ def kernel_rooflines(prof, cells, nv, mixed, nt=0):
     return out
 
 
+def rel_linf(a, b):
+    """max |a - b| over max |b| per conserved variable (interior cells)"""
+    I = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
+    d = np.abs(a[I] - b[I]).max(axis=(0, 2, 3, 4))
+    return d / np.abs(b[I]).max(axis=(0, 2, 3, 4))
+
+
 def rel_l2(a, b):
     """relative L2 error of a against b per conserved variable (interior cells)"""
     I = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
@@ -462,6 +469,7 @@ def main():
     rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup)
     # the same seeded run in both forms: accuracy of the mixed form vs FP64
     l2 = rel_l2(rmx["u"], r64["u"])
+    linf = rel_linf(rmx["u"], r64["u"])
     # what FP32 rounding of the input alone does to the FP64 result: the
     # noisy atmosphere amplifies any perturbation, so this is the floor any
     # FP32 computation sits on
@@ -528,6 +536,8 @@ def main():
                  "dt_last_s": r64["dt"], "kernels": kernels64},
         "mixed_over_fp64": value / value64,
         "mixed_rel_l2_vs_fp64": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"), l2)},
+        "mixed_rel_linf_vs_fp64": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"),
+                                                               linf)},
         "mixed_rel_l2_max": float(l2.max()),
         "fp64_input_rounding_floor": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"),
                                                                  floor)},
