@@ -12,6 +12,8 @@
 //   HLLC:   t_l = P + t_l', cp = P + cp', sl + sr + sm = 1
 // Shallow water instead takes each face about its left cell's FP64 state
 // (roe_sw_delta, lmars_sw_delta below).
+// Unlike the FP64 path, the FP32 WENO5 uses FMAs: the mixed form is judged by
+// its error bound, not by reproducing the reference's roundings.
 #pragma once
 
 #include "exo_device.hpp"
@@ -26,21 +28,25 @@ EXO_HD float frcp(float x) {
 #endif
 }
 
-// WENO5 of exo_device.hpp in FP32 with hardware reciprocals for the weights
+// WENO5 of exo_device.hpp in FP32 with hardware reciprocals for the weights,
+// its products and sums written as explicit FMAs.  Explicit, because the edge
+// kernel and the flux kernels reconstruct the same panel-edge value and must
+// agree bit for bit (the two panels' edge fluxes then agree and mass is
+// conserved); compiler contraction could fuse differently in each kernel.
 EXO_HD float weno5f(float phim2, float phim1, float phi, float phip1, float phip2) {
-  float p0 = (1.f / 3.f) * phi + (5.f / 6.f) * phim1 - (1.f / 6.f) * phim2;
-  float p1 = (-1.f / 6.f) * phip1 + (5.f / 6.f) * phi + (1.f / 3.f) * phim1;
-  float p2 = (1.f / 3.f) * phip2 - (7.f / 6.f) * phip1 + (11.f / 6.f) * phi;
-  float t0 = phi - 2.f * phim1 + phim2, s0 = 3.f * phi - 4.f * phim1 + phim2;
-  float t1 = phip1 - 2.f * phi + phim1, s1 = phip1 - phim1;
-  float t2 = phip2 - 2.f * phip1 + phi, s2 = phip2 - 4.f * phip1 + 3.f * phi;
-  float b0 = (13.f / 12.f) * t0 * t0 + .25f * s0 * s0 + 1e-6f;
-  float b1 = (13.f / 12.f) * t1 * t1 + .25f * s1 * s1 + 1e-6f;
-  float b2 = (13.f / 12.f) * t2 * t2 + .25f * s2 * s2 + 1e-6f;
+  float p0 = fmaf(1.f / 3.f, phi, fmaf(5.f / 6.f, phim1, (-1.f / 6.f) * phim2));
+  float p1 = fmaf(-1.f / 6.f, phip1, fmaf(5.f / 6.f, phi, (1.f / 3.f) * phim1));
+  float p2 = fmaf(1.f / 3.f, phip2, fmaf(-7.f / 6.f, phip1, (11.f / 6.f) * phi));
+  float t0 = fmaf(-2.f, phim1, phi + phim2), s0 = fmaf(3.f, phi, fmaf(-4.f, phim1, phim2));
+  float t1 = fmaf(-2.f, phi, phip1 + phim1), s1 = phip1 - phim1;
+  float t2 = fmaf(-2.f, phip1, phip2 + phi), s2 = fmaf(3.f, phi, fmaf(-4.f, phip1, phip2));
+  float b0 = fmaf((13.f / 12.f) * t0, t0, fmaf(.25f * s0, s0, 1e-6f));
+  float b1 = fmaf((13.f / 12.f) * t1, t1, fmaf(.25f * s1, s1, 1e-6f));
+  float b2 = fmaf((13.f / 12.f) * t2, t2, fmaf(.25f * s2, s2, 1e-6f));
   float a0 = .3f * frcp(b0 * b0);
   float a1 = .6f * frcp(b1 * b1);
   float a2 = .1f * frcp(b2 * b2);
-  return (a0 * p0 + a1 * p1 + a2 * p2) * frcp(a0 + a1 + a2);
+  return fmaf(a0, p0, fmaf(a1, p1, a2 * p2)) * frcp(a0 + a1 + a2);
 }
 
 // q[IDN] full density (geopotential for shallow water), q[IVX..IVZ]
