@@ -1,0 +1,64 @@
+"""The benchmark configurations at their full sizes.
+
+* Shallow water 6 x 48 x 48 (3 layers), Roe and LMARS, Coriolis, 100 steps:
+  the reference's own CPU-runnable case, bit for bit against the oracle.
+* The 3D atmosphere 6 x 96 x 96 x 64 (LMARS, implicit 5x5), two FP64 steps bit for
+  bit against the oracle.
+* The large rotating atmosphere 6 x 384 x 384 x 128 with two tracers in the
+  mixed form, where the oracle is too slow: size-independent properties (mass
+  conserved to rounding, a finite state with tracers, a positive time step)."""
+import numpy as np
+import pytest
+
+from helpers import R_EARTH, interior, sw_config
+
+pytestmark = pytest.mark.gpu
+SEED = 20261202
+
+
+@pytest.mark.parametrize("rsolver", [0, 1])
+def test_shallow_water_48_100_steps(ex, orc, rsolver):
+    cfg = sw_config(ex, nx=48, rsolver=rsolver, nx1=3)
+    u0, _ = ex.init_state(cfg, 0, SEED, ())
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    o = orc.Oracle(cfg)
+    o.upload(u0)
+    assert g.step(100) == o.step(100)
+    assert np.array_equal(interior(g.download()[0], 3), interior(o.download()[0], 3))
+    g.close()
+    o.close()
+
+
+def test_atmosphere_96x64_two_steps(ex, orc):
+    cfg = ex.make_config(nx=96, nx1=64, eos=ex.EOS_IDEAL, rsolver=ex.RS_LMARS, implicit_flag=9,
+                         cfl_x1=1, x1min=R_EARTH, x1max=R_EARTH + 30000.0, gamma=1.4,
+                         grav=-9.81, omega=7.292e-5, cfl=0.4)
+    u0, _ = ex.init_state(cfg, 1, SEED, (250.0, 1.0e5, 287.0, 0.01, 5.0))
+    g = ex.Solver(cfg)
+    g.upload(u0)
+    o = orc.Oracle(cfg)
+    o.upload(u0)
+    assert g.step(2) == o.step(2)
+    assert np.array_equal(interior(g.download()[0], 64), interior(o.download()[0], 64))
+    g.close()
+    o.close()
+
+
+def test_large_rotating_atmosphere_conserves(ex):
+    cfg = ex.make_config(nx=384, nx1=128, eos=ex.EOS_IDEAL, rsolver=ex.RS_HLLC, implicit_flag=9,
+                         ntracer=2, coriolis=ex.COR_3D, cfl_x1=1, x1min=R_EARTH,
+                         x1max=R_EARTH + 30000.0, gamma=1.4, grav=-9.81, omega=7.292e-5,
+                         cfl=0.4, precision=ex.PREC_MIXED)
+    u0, s0 = ex.init_state(cfg, 2, SEED, (300.0, 6.5e-3, 200.0, 1.0e5, 287.0, 0.5, 3.0e-3, 0.5))
+    g = ex.Solver(cfg)
+    g.upload(u0, s0)
+    del u0, s0
+    m0 = g.sums()[0]
+    dt = g.step(3)
+    m1 = g.sums()[0]
+    assert dt > 0.0
+    assert abs(m1 - m0) <= 1e-13 * abs(m0), (m1 - m0) / m0
+    u, s = g.download()
+    assert np.isfinite(interior(u, 128)).all() and np.isfinite(interior(s, 128)).all()
+    g.close()
