@@ -108,6 +108,9 @@ struct StateT {
                                         // the reconstructed variable's reference,
                                         // [2nc1,3nc1) rho_ref
   PTR(double) part;                     // per-block compensated partial sums [block][5][2]
+  PTR(float) fw;                        // mixed ideal gas: FP32 perturbation primitives
+                                        // (w - level reference) and tracer r, [nv + nt]
+                                        // cell arrays, written with w by k_prim
   PTR(int) bad;                         // set by the primitive conversion of the stage
                                         // when a density or pressure is negative or NaN
 };
@@ -150,6 +153,7 @@ __device__ __forceinline__ DState to_global(const State &h) {
   s.edge = glb(h.edge); s.coef = glb(h.coef); s.dtbits = glb(h.dtbits); s.dtv = glb(h.dtv);
   s.fedge = glb(h.fedge); s.psf = glb(h.psf); s.lref = glb(h.lref); s.part = glb(h.part);
   s.bad = glb(h.bad);
+  s.fw = glb(h.fw);
   return s;
 }
 
@@ -277,12 +281,24 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
     if (mirror && qb <= NG) st.w[g.c(n, p, k, g.is - qb, j)] = m;
     if (mirror && qt <= NG) st.w[g.c(n, p, k, g.ie + qt, j)] = m;
   }
+  // the mixed ideal gas's face and edge kernels read these values as FP32
+  // perturbations about the level reference of pressure and density
+  if (st.fw) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      double v = wv[n];
+      if (n == IPR) v -= st.lref[g.nc1 + i];
+      else if (n == IDN) v -= st.lref[2 * g.nc1 + i];
+      st.fw[g.c(n, p, k, i, j)] = (float)v;
+    }
+  }
   for (int n = 0; n < g.nt; ++n) {
     double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / rho_c);
     rr = rr < sfloor ? sfloor : rr;
     st.r[g.c(n, p, k, i, j)] = rr;
     if (qb <= NG) st.r[g.c(n, p, k, g.is - qb, j)] = rr;
     if (qt <= NG) st.r[g.c(n, p, k, g.ie + qt, j)] = rr;
+    if (st.fw) st.fw[g.c(g.nv + n, p, k, i, j)] = (float)rr;
   }
 }
 
@@ -1611,19 +1627,8 @@ __global__ void __launch_bounds__(64 * X1W) k_flux_x1_mix(const Inst *__restrict
   }
 }
 
-// cell value as an FP32 perturbation about the level reference
-__device__ __forceinline__ float pert(const DGrid &g, const DState &st, int n, int p, int k,
-                                      int i, int j) {
-  double v = st.w[g.c(n, p, k, i, j)];
-  if (g.eos == EOS_SW) {
-    if (n == IDN) v -= st.lref[g.nc1 + i];
-  } else {
-    if (n == IPR) v -= st.lref[g.nc1 + i];
-    else if (n == IDN) v -= st.lref[2 * g.nc1 + i];
-  }
-  return (float)v;
-}
-
+// mixed ideal gas: the interior-side WENO5 state of a panel-edge face, moved
+// into the neighbour's frame and slot
 __device__ __forceinline__ void edge_mix_body(const DGrid &g, const DState &st, int lb,
                                               const ADDR(1) EdgeDesc *__restrict__ ed, int nedge) {
 #pragma clang fp contract(fast)
@@ -1634,26 +1639,28 @@ __device__ __forceinline__ void edge_mix_body(const DGrid &g, const DState &st, 
   EdgeDesc d = *(const EdgeDesc *)(ed + e);
   int p = d.src_panel, k = d.src_k, j = d.src_j;
   float q[5] = {0, 0, 0, 0, 0};
+  // the FP32 perturbations k_prim wrote
+  auto FW = [&](int n, int kk, int ii, int jj) -> float { return st.fw[g.c(n, p, kk, ii, jj)]; };
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
     if (g.skipv(n)) continue;
     float v;
     if (d.src_dir == 0)
-      v = weno5f(pert(g, st, n, p, k, i, j - 2), pert(g, st, n, p, k, i, j - 1),
-                 pert(g, st, n, p, k, i, j), pert(g, st, n, p, k, i, j + 1),
-                 pert(g, st, n, p, k, i, j + 2));
+      v = weno5f(FW(n, k, i, j - 2), FW(n, k, i, j - 1),
+                 FW(n, k, i, j), FW(n, k, i, j + 1),
+                 FW(n, k, i, j + 2));
     else if (d.src_dir == 1)
-      v = weno5f(pert(g, st, n, p, k, i, j + 1), pert(g, st, n, p, k, i, j),
-                 pert(g, st, n, p, k, i, j - 1), pert(g, st, n, p, k, i, j - 2),
-                 pert(g, st, n, p, k, i, j - 3));
+      v = weno5f(FW(n, k, i, j + 1), FW(n, k, i, j),
+                 FW(n, k, i, j - 1), FW(n, k, i, j - 2),
+                 FW(n, k, i, j - 3));
     else if (d.src_dir == 2)
-      v = weno5f(pert(g, st, n, p, k - 2, i, j), pert(g, st, n, p, k - 1, i, j),
-                 pert(g, st, n, p, k, i, j), pert(g, st, n, p, k + 1, i, j),
-                 pert(g, st, n, p, k + 2, i, j));
+      v = weno5f(FW(n, k - 2, i, j), FW(n, k - 1, i, j),
+                 FW(n, k, i, j), FW(n, k + 1, i, j),
+                 FW(n, k + 2, i, j));
     else
-      v = weno5f(pert(g, st, n, p, k + 1, i, j), pert(g, st, n, p, k, i, j),
-                 pert(g, st, n, p, k - 1, i, j), pert(g, st, n, p, k - 2, i, j),
-                 pert(g, st, n, p, k - 3, i, j));
+      v = weno5f(FW(n, k + 1, i, j), FW(n, k, i, j),
+                 FW(n, k - 1, i, j), FW(n, k - 2, i, j),
+                 FW(n, k - 3, i, j));
     q[n] = v;
   }
   double U, V, vx, vy;
@@ -1744,7 +1751,7 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
       int ks = dir == 1 ? k : k - 3 + s, js = dir == 1 ? j - 3 + s : j;
-      v[s] = sw ? (float)(st.w[g.c(n, p, ks, i, js)] - B[n]) : pert(g, st, n, p, ks, i, js);
+      v[s] = sw ? (float)(st.w[g.c(n, p, ks, i, js)] - B[n]) : st.fw[g.c(n, p, ks, i, js)];
     }
     wl[n] = weno5f(v[4], v[3], v[2], v[1], v[0]);
     wr[n] = weno5f(v[1], v[2], v[3], v[4], v[5]);
@@ -1815,8 +1822,9 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
     if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
   for (int n = 0; n < g.nt; ++n) {
     float r[6];
+    int nr = g.nv + n;  // the tracer's FP32 copy
     for (int s = 0; s < 6; ++s)
-      r[s] = (float)(dir == 1 ? st.r[g.c(n, p, k, i, j - 3 + s)] : st.r[g.c(n, p, k - 3 + s, i, j)]);
+      r[s] = dir == 1 ? st.fw[g.c(nr, p, k, i, j - 3 + s)] : st.fw[g.c(nr, p, k - 3 + s, i, j)];
     float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
     float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
     st.fsflx[dir][g.f(n, p, k, i, j)] = m >= 0.f ? m * rl : m * rr;
@@ -2222,10 +2230,12 @@ size_t footprint_bytes(const Setup &s) {
     }
     add(edge * g.nv * sizeof(float));
     add(face);
+    add(g.vstride * sizeof(float) * (g.nv + g.nt));  // fw
   }
   if (mixed) add(3 * (size_t)g.nc1 * sizeof(double));
   add(sizeof(unsigned long long));
   add(2 * sizeof(double));
+  add(sizeof(int));
   add((size_t)grid1((size_t)6 * g.N * g.N * g.nx1) * 10 * sizeof(double));
   add(5 * sizeof(double));
   add(sizeof(Inst));
@@ -2298,6 +2308,7 @@ void init_handle(exo_handle *h) {
     }
     st.fedge = (float *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(float));
     st.psf = (double *)h->dalloc(g.fvstride * sizeof(double));
+    st.fw = (float *)h->dalloc((size_t)(g.nv + g.nt) * g.vstride * sizeof(float));
   }
   if (h->mixed) st.lref = (double *)h->dalloc((size_t)3 * g.nc1 * sizeof(double));
   st.dtbits = (unsigned long long *)h->dalloc(sizeof(unsigned long long));
@@ -2820,7 +2831,8 @@ void handle_graph(exo_handle *h, double dt_fixed) {
 // primitive state, here with the passive scalars and the time-step state,
 // copied within HBM
 size_t snap_doubles(const exo_handle *h) {
-  return (size_t)(2 * h->g.nv + 2 * h->g.nt) * h->g.vstride + 2;
+  size_t fw = h->st.fw ? ((size_t)(h->g.nv + h->g.nt) * h->g.vstride + 1) / 2 : 0;
+  return (size_t)(2 * h->g.nv + 2 * h->g.nt) * h->g.vstride + 2 + fw;
 }
 
 void snap_copy(exo_handle *h, bool save) {
@@ -2838,6 +2850,11 @@ void snap_copy(exo_handle *h, bool save) {
     cp(h->st.r, (size_t)g.nt * g.vstride);
   }
   cp(h->st.dtv, 2);
+  if (h->st.fw) {  // the FP32 perturbations the face kernels read
+    size_t bytes = (size_t)(g.nv + g.nt) * g.vstride * sizeof(float);
+    HIPCHK(hipMemcpyAsync(save ? (void *)d : (void *)h->st.fw, save ? (void *)h->st.fw : (void *)d,
+                          bytes, hipMemcpyDeviceToDevice, h->stream));
+  }
 }
 
 void save_states(exo_handle *h) {

@@ -148,3 +148,25 @@ def test_output_before_upload_is_refused(ex):
     with pytest.raises(ex.ExoError):
         g.output_fields("prim")
     g.close()
+
+
+def test_w92_run_files_match_oracle_files(ex, orc, tmp_path):
+    """Integration in the reference's style (run, write the per-block files,
+    compare them with a known-good set by an L2 norm): a W92-like shallow-water
+    sphere run 20 steps on the device and written by exo_write_netcdf, against
+    the same run of the oracle written from its fields by the host writer."""
+    cfg = sw_config(ex, nx=24, rsolver=0, nx1=3)
+    g, o = run_pair(ex, orc, cfg, 0, 20)
+    for var in ("prim", "uov"):
+        gf = g.write_netcdf(str(tmp_path / "gpu"), var, time=3600.0, file_number=1)
+        ex.write_netcdf_fields(cfg, str(tmp_path / "ref"), var, expected(o, cfg, var, False),
+                               time=3600.0, file_number=1)
+        of = ex.output_file_names(str(tmp_path / "ref"), var, 1)
+        for a, b in zip(gf, of):
+            with netcdf_file(a, "r", mmap=False) as fa, netcdf_file(b, "r", mmap=False) as fb:
+                assert list(fa.variables) == list(fb.variables)
+                for n in fa.variables:
+                    x, y = fa.variables[n].data, fb.variables[n].data
+                    assert np.sqrt(np.sum((x.astype(np.float64) - y) ** 2)) == 0.0, (a, n)
+    g.close()
+    o.close()
