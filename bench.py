@@ -445,6 +445,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--requests", type=int, default=16)
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="only the 256-instance shallow-water batch, both forms (for profiling)")
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
@@ -465,6 +467,11 @@ def main():
         sys.exit(2)
     import exocubed_amd as ex
 
+    if args.only_ensemble:
+        e64, _ = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)
+        emx, _ = ensemble(ex, ex.PREC_MIXED, args.ensemble_steps)
+        print(json.dumps({"ensemble": {"fp64": e64, "mixed": emx}}))
+        return
     r64 = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup)
     rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup)
     # the same seeded run in both forms: accuracy of the mixed form vs FP64
