@@ -15,7 +15,9 @@
  *                              ghost remap of constant / smooth fields,
  *                              block-Thomas vs dense solve, Riemann consistency,
  *                              W92 user outputs (known answers, rigid lat-lon
- *                              map, wind speed and direction)
+ *                              map, wind speed and direction), the redo path
+ *                              (plain stepping without failures; a redone step
+ *                              is a step at dt / 2^k from the restored state)
  *   parity unpinned          : Athena-side semantics not in the reference tree
  *                              (AddFluxDivergence, SSPRK3 weight order,
  *                              NewBlockTimeStep, reflecting BC fill) -- restated
