@@ -25,7 +25,7 @@ KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "upd
 # launch kinds of the persistent-width tables (exo_tune / exo_batch_tune)
 LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
                 "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one", "dt",
-                "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost")
+                "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost", "implicit3")
 
 
 class Config(C.Structure):

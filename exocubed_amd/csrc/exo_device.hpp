@@ -501,6 +501,106 @@ EXO_HD M5 m5_inv(const M5 &A) {
   return X;
 }
 
+// The same algebra on the 3x3 blocks (density, x1 momentum, energy) of the
+// partial column correction (partial_correction.cpp).
+struct M3 {
+  double a[3][3];
+};
+struct V3 {
+  double a[3];
+};
+
+EXO_HD M3 m3_mul(const M3 &A, const M3 &B) {
+  M3 C;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double t = A.a[r][0] * B.a[0][c];
+#pragma unroll
+      for (int k = 1; k < 3; ++k) t = fma(A.a[r][k], B.a[k][c], t);
+      C.a[r][c] = t;
+    }
+  return C;
+}
+
+EXO_HD V3 m3_mulv(const M3 &A, const V3 &x) {
+  V3 y;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double t = A.a[r][0] * x.a[0];
+#pragma unroll
+    for (int k = 1; k < 3; ++k) t = fma(A.a[r][k], x.a[k], t);
+    y.a[r] = t;
+  }
+  return y;
+}
+
+EXO_HD M3 m3_inv(const M3 &A) {
+  double lu[3][3], rd[3];
+  int piv[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lu[r][c] = A.a[r][c];
+    piv[r] = r;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    int p = c;
+    double best = fabs(lu[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < 3; ++r)
+      if (fabs(lu[r][c]) > best) {
+        best = fabs(lu[r][c]);
+        p = r;
+      }
+#pragma unroll
+    for (int r = c + 1; r < 3; ++r) {
+      if (p == r) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          double t = lu[c][q];
+          lu[c][q] = lu[r][q];
+          lu[r][q] = t;
+        }
+        int t = piv[c];
+        piv[c] = piv[r];
+        piv[r] = t;
+      }
+    }
+    double yc = 1.0 / lu[c][c];
+    rd[c] = yc;
+#pragma unroll
+    for (int r = c + 1; r < 3; ++r) {
+      double l = div_rn(lu[r][c], lu[c][c], yc);
+      lu[r][c] = l;
+#pragma unroll
+      for (int q = c + 1; q < 3; ++q) lu[r][q] = fma(-l, lu[c][q], lu[r][q]);
+    }
+  }
+  M3 X;
+#pragma unroll
+  for (int col = 0; col < 3; ++col) {
+    double y[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      double t = piv[r] == col ? 1.0 : 0.0;
+#pragma unroll
+      for (int q = 0; q < r; ++q) t = fma(-lu[r][q], y[q], t);
+      y[r] = t;
+    }
+#pragma unroll
+    for (int r = 2; r >= 0; --r) {
+      double t = y[r];
+#pragma unroll
+      for (int q = r + 1; q < 3; ++q) t = fma(-lu[r][q], X.a[q][col], t);
+      X.a[r][col] = div_rn(t, lu[r][r], rd[r]);
+    }
+  }
+  return X;
+}
+
 // flux_decomposition.hpp:34-61 RoeAverage (NVAPOR = 0)
 EXO_HD void roe_average(double *prim, double gm1, const double *wl, const double *wr) {
   double sqrtdl = sqrt(wl[IDN]), sqrtdr = sqrt(wr[IDN]);

@@ -1235,6 +1235,182 @@ __global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Wo
   }
 }
 
+// ImplicitSolver::PartialCorrection on an x1 column (partial_correction.cpp:
+// 26-177): the block-tridiagonal system of the (density, x1 momentum, energy)
+// rows and columns of the same |A| and flux Jacobians, gravity in Phi, the
+// reflecting ends diag(1, -1, 1).  Only those three tendencies are replaced;
+// the transverse momenta take their explicit tendency.  Ends with u += du.
+constexpr int SUB3[3] = {IDN, IVX, IEN};
+
+__device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st, int lb,
+                                               double beta, double *geo) {
+  for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
+    geo[q] = g.x1f[q];
+    if (q < g.nc1) {
+      geo[IMPL_MAXNC1 + 1 + q] = g.x1v[q];
+      geo[2 * (IMPL_MAXNC1 + 1) + q] = g.dx1f[q];
+    }
+  }
+  __syncthreads();
+  int N = g.N;
+  int nbx = (N * N + 63) / 64;
+  int col = (lb % nbx) * 64 + threadIdx.x;
+  int p = lb / nbx;
+  if (col >= N * N) return;
+  int k = g.ks + col / N, j = g.js + col % N;
+  int is = g.is, ie = g.ie;
+  double dt = beta * st.dtv[0];
+  double gamma = g.gamma, grav = g.grav;
+  double cth = g.T(T_COSC, p, k, j), sth = g.T(T_SINC, p, k, j);
+  double dx2a = g.T(T_DX2A, p, k, j), dx3a = g.T(T_DX3A, p, k, j);
+  double a23 = dx2a * dx3a;
+  const double *x1f = geo, *x1v = geo + IMPL_MAXNC1 + 1, *dx1f = geo + 2 * (IMPL_MAXNC1 + 1);
+  const ADDR(1) double *__restrict__ W = st.w;
+  const ADDR(1) double *__restrict__ DU = st.du;
+  ADDR(1) double *__restrict__ U = st.u;
+  // coefficient scratch, [block][level][pair][lane]: the 12 doubles of a level
+  // (a^-1 c row-major, then delta) as 6 pairs of 64 lanes x 16 B
+  typedef double dpair __attribute__((ext_vector_type(2)));
+  ADDR(1) dpair *__restrict__ CFP =
+      (ADDR(1) dpair *)st.coef + (size_t)lb * g.nx1 * 6 * 64 + threadIdx.x;
+  auto cfp = [&](int i, int q2) -> ADDR(1) dpair * {
+    return CFP + ((size_t)(i - is) * 6 + q2) * 64;
+  };
+  size_t vs = g.vstride, jp = g.jp;
+  size_t c0 = g.c(0, p, k, 0, j);
+  auto wload = [&](int i, double *w) {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) w[n] = W[n * vs + c0 + (size_t)i * jp];
+    w[IVY] += w[IVZ] * cth;
+    w[IVZ] *= sth;
+  };
+  double gm1c = (gamma - 1.) * 1. / 1.;
+  double gm1 = 0.5 * (gm1c + gm1c);
+  double k1 = (gm1c + 1) / gm1c;
+  auto erow = [&](const double *w) { return k1 * w[IPR] / w[IDN]; };
+  double wm[5], w0[5], wp[5], prim[5];
+  wload(is - 1, wm);
+  wload(is, w0);
+  roe_average(prim, gm1, wm, w0);
+  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  M5 Am = abs_a(prim, cs, gm1);
+  double em = erow(wm), e0 = erow(w0);
+  double ydt = 1. / dt;
+  M3 Aprev;
+  V3 dprev;
+  for (int i = is; i <= ie; ++i) {
+    wload(i + 1, wp);
+    double ep = erow(wp);
+    V3 rhs;
+#pragma unroll
+    for (int n = 0; n < 3; ++n)
+      rhs.a[n] = div_rn(DU[SUB3[n] * vs + c0 + (size_t)i * jp], dt, ydt);
+    double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
+    double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
+    double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
+    double v2 = 2. * v, yv2 = 1. / v2;
+    roe_average(prim, gm1, w0, wp);
+    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    M5 Ap = abs_a(prim, cs, gm1);
+    M5 Jm = jac_e(gm1c, wm, em), J0 = jac_e(gm1c, w0, e0), Jp = jac_e(gm1c, wp, ep);
+    M3 b, c, T;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        int R = SUB3[r], Q = SUB3[q];
+        b.a[r][q] = div_rn(-(Am.a[R][Q] + Jm.a[R][Q]) * aleft, v2, yv2);
+        c.a[r][q] = div_rn(-(Ap.a[R][Q] - Jp.a[R][Q]) * aright, v2, yv2);
+      }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        int R = SUB3[r], Q = SUB3[q];
+        double dtq = r == q ? ydt : 0.;
+        double phi = (r == 1 && q == 0) || (r == 2 && q == 1) ? grav : 0.;
+        double a = div_rn(Am.a[R][Q] * aleft + Ap.a[R][Q] * aright +
+                              (aright - aleft) * J0.a[R][Q],
+                          v2, yv2) +
+                   dtq - phi;
+        if (i == is) a += b.a[r][q] * (q == 1 ? -1. : 1.);  // reflecting bottom
+        if (i == ie) a += c.a[r][q] * (q == 1 ? -1. : 1.);  // reflecting top
+        T.a[r][q] = a;
+      }
+    V3 tv = rhs;
+    if (i > is) {
+      M3 ba = m3_mul(b, Aprev);
+      V3 bd = m3_mulv(b, dprev);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) T.a[r][q] = T.a[r][q] - ba.a[r][q];
+        tv.a[r] = rhs.a[r] - bd.a[r];
+      }
+    }
+    M3 ainv = m3_inv(T);
+    V3 delta = m3_mulv(ainv, tv);
+    Aprev = m3_mul(ainv, c);
+    dprev = delta;
+    double cv[12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cv[r * 3 + q] = Aprev.a[r][q];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) cv[9 + n] = delta.a[n];
+#pragma unroll
+    for (int q2 = 0; q2 < 6; ++q2) *cfp(i, q2) = dpair{cv[2 * q2], cv[2 * q2 + 1]};
+    Am = Ap;
+    em = e0;
+    e0 = ep;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      wm[n] = w0[n];
+      w0[n] = wp[n];
+    }
+  }
+  // backward substitution, then u += du: the corrected (density, x1 momentum,
+  // energy) and the explicit transverse momenta
+  V3 dnext = dprev;
+  for (int i = ie; i >= is; --i) {
+    V3 d = dnext;
+    if (i < ie) {
+      double cv[12];
+#pragma unroll
+      for (int q2 = 0; q2 < 6; ++q2) {
+        dpair t = *cfp(i, q2);
+        cv[2 * q2] = t.x;
+        cv[2 * q2 + 1] = t.y;
+      }
+      M3 A;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) A.a[r][q] = cv[r * 3 + q];
+      V3 ad = m3_mulv(A, dnext);
+#pragma unroll
+      for (int n = 0; n < 3; ++n) d.a[n] = cv[9 + n] - ad.a[n];
+    }
+    size_t ci = c0 + (size_t)i * jp;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) U[SUB3[n] * vs + ci] += d.a[n];
+    U[IVY * vs + ci] += DU[IVY * vs + ci];
+    U[IVZ * vs + ci] += DU[IVZ * vs + ci];
+    dnext = d;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_implicit3(const Inst *__restrict__ I_, Work w,
+                                                  double beta) {
+  __shared__ double geo[3 * (IMPL_MAXNC1 + 1)];
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    implicit3_body(g, st, lb, beta, geo);
+    __syncthreads();
+  }
+}
+
 
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
 __device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, int lb) {
@@ -1967,7 +2143,9 @@ enum {
   K_UPD, K_UPD_MIX, K_IMPL, K_ADD_DU, K_CELLS, K_ONE, K_DT, K_EFLX, K_FACE_MIX_SW,
   // within a step: the full primitive conversion for the ideal gas, the ghost
   // strips for shallow water (its stage update writes the interior)
-  K_PRIM_REST, K_PRIM_GHOST, K_NKIND
+  K_PRIM_REST, K_PRIM_GHOST,
+  // the partial (3x3) column correction
+  K_IMPL3, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2088,10 +2266,12 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
-    case K_IMPL:
-      if (!ideal || !g.implicit) return 0;
-      return (int)grid1(cols, 64) * 6;
-    case K_ADD_DU: return ideal && !g.implicit ? (int)grid1(cells) : 0;
+    // implicit_flag bit 1 solves the x1 columns, bit 8 with the full 5x5
+    // system (solve_implicit_3d.cpp:102-135); without bit 1, u += du
+    case K_IMPL: return ideal && (g.implicit & 1) && (g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
+    case K_IMPL3:
+      return ideal && (g.implicit & 1) && !(g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
+    case K_ADD_DU: return ideal && !(g.implicit & 1) ? (int)grid1(cells) : 0;
     case K_CELLS: return (int)grid1(cells);
     case K_ONE: return 1;
     case K_DT: return (int)grid1(cells, BLK * DTC);
@@ -2134,7 +2314,7 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     bool need_x1 = L.nblk[K_X1] > 0 || L.nblk[K_X1_MIX] > 0;
     for (const exo_handle *h : hs)
-      if (h->g.nc1 > IMPL_MAXNC1 && h->g.eos == EXO_EOS_IDEAL && h->g.implicit)
+      if (h->g.nc1 > IMPL_MAXNC1 && h->g.eos == EXO_EOS_IDEAL && (h->g.implicit & 1))
         throw std::string("nx1 too large for the implicit column solve (nc1 <= ") +
             std::to_string(IMPL_MAXNC1) + ")";
     if (need_x1 && L.lds_x1 > (size_t)maxlds)
@@ -2222,7 +2402,7 @@ size_t footprint_bytes(const Setup &s) {
       for (int d = 0; d < 3; ++d) add(face * g.nt);
     add(edge * g.nv * sizeof(double));
   }
-  if (g.eos == EXO_EOS_IDEAL && g.implicit) add(cell * 30);
+  if (g.eos == EXO_EOS_IDEAL && (g.implicit & 1)) add(cell * (g.implicit & 8 ? 30 : 12));
   if (f32) {
     for (int d = 0; d < 3; ++d) {
       add(g.fvstride * sizeof(float) * g.nv);
@@ -2294,11 +2474,13 @@ void init_handle(exo_handle *h) {
       for (int d = 0; d < 3; ++d) st.sflx[d] = (double *)h->dalloc(h->face_bytes(g.nt));
     st.edge = (double *)h->dalloc((size_t)g.nv * 6 * 4 * g.N * g.nx1 * sizeof(double));
   }
-  if (g.eos == EXO_EOS_IDEAL && g.implicit) {
-    // k_implicit's [block][level][15 pairs][64 lanes] scratch within 30 cell arrays
-    size_t need = (size_t)6 * ((g.N * g.N + 63) / 64) * g.nx1 * 30 * 64;
-    if (need > (size_t)30 * g.vstride) throw std::string("implicit scratch layout does not fit");
-    st.coef = (double *)h->dalloc(h->cells_bytes(30));
+  if (g.eos == EXO_EOS_IDEAL && (g.implicit & 1)) {
+    // the column solve's [block][level][pair][64 lanes] scratch: 15 pairs for
+    // the 5x5 system (k_implicit), 6 for the 3x3 (k_implicit3)
+    int per = g.implicit & 8 ? 30 : 12;
+    size_t need = (size_t)6 * ((g.N * g.N + 63) / 64) * g.nx1 * per * 64;
+    if (need > (size_t)per * g.vstride) throw std::string("implicit scratch layout does not fit");
+    st.coef = (double *)h->dalloc(h->cells_bytes(per));
   }
   if (f32) {
     size_t fb = (size_t)g.fvstride * sizeof(float);
@@ -2362,6 +2544,7 @@ void launch_update(Launcher &L, int stage, int prim) {
 
 void launch_implicit(Launcher &L, double beta) {
   launch(L, K_IMPL, KID_IMPLICIT, k_implicit, 64, 0, beta);
+  launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3, 64, 0, beta);
   launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
   HIPCHK(hipGetLastError());
 }

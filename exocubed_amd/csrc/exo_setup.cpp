@@ -331,8 +331,9 @@ std::string build_setup(const exo_config &cfg, Setup *s) {
     return "shallow-water EOS needs a shallow-water Riemann solver";
   if (cfg.eos == EXO_EOS_IDEAL && cfg.rsolver < EXO_RS_HLLC)
     return "ideal-gas EOS needs the HLLC or LMARS Riemann solver";
-  if (cfg.implicit_flag != 0 && cfg.implicit_flag != 9)
-    return "implicit_flag must be 0 or 9 (x1 full 5x5 correction)";
+  // bit 1: x1 column solve; bit 8: the full 5x5 system instead of the partial
+  // 3x3 one (solve_implicit_3d.cpp:102-135).  The x2/x3 bits are not supported.
+  if (cfg.implicit_flag & ~9) return "implicit_flag must be 0, 1, 8 or 9 (x1 column correction)";
   if (cfg.eos == EXO_EOS_SHALLOW_YZ && cfg.implicit_flag != 0)
     return "the implicit solve needs the ideal-gas EOS";
   if (cfg.eos == EXO_EOS_IDEAL && cfg.nx1 < 3) return "ideal gas needs nx1 >= 3";

@@ -54,7 +54,11 @@ typedef struct exo_config {
   int nx1;           /* radial cells */
   int eos;           /* EXO_EOS_* */
   int rsolver;       /* EXO_RS_* */
-  int implicit_flag; /* 0 or 9 (<hydro> implicit_flag, solve_implicit_3d.cpp:102-135) */
+  int implicit_flag; /* <hydro> implicit_flag (solve_implicit_3d.cpp:102-135): bit 1 solves
+                        the x1 columns, with the full 5x5 system when bit 8 is set
+                        (full_correction.cpp), else the partial 3x3 one
+                        (partial_correction.cpp); 0 and 8: u += du.  The x2/x3 bits
+                        (2, 4) are rejected. */
   int ntracer;       /* passive scalars (upwinded by the mass flux) */
   int coriolis;      /* EXO_COR_* */
   int cfl_x1;        /* 1: radial widths enter the CFL minimum */

@@ -1316,6 +1316,68 @@ static m5 m5_inv(const m5 *A) {
   return X;
 }
 
+/* the same algebra on the 3x3 blocks of the partial correction */
+typedef struct { double a[3][3]; } m3;
+typedef struct { double a[3]; } v3;
+
+static m3 m3_mul(const m3 *A, const m3 *B) {
+  m3 C;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double t = A->a[r][0] * B->a[0][c];
+      for (int k = 1; k < 3; ++k) t = fma(A->a[r][k], B->a[k][c], t);
+      C.a[r][c] = t;
+    }
+  return C;
+}
+
+static v3 m3_mulv(const m3 *A, const v3 *x) {
+  v3 y;
+  for (int r = 0; r < 3; ++r) {
+    double t = A->a[r][0] * x->a[0];
+    for (int k = 1; k < 3; ++k) t = fma(A->a[r][k], x->a[k], t);
+    y.a[r] = t;
+  }
+  return y;
+}
+
+static m3 m3_inv(const m3 *A) {
+  double lu[3][3];
+  int piv[3];
+  memcpy(lu, A->a, sizeof(lu));
+  for (int r = 0; r < 3; ++r) piv[r] = r;
+  for (int c = 0; c < 3; ++c) {
+    int p = c;
+    double best = fabs(lu[c][c]);
+    for (int r = c + 1; r < 3; ++r)
+      if (fabs(lu[r][c]) > best) { best = fabs(lu[r][c]); p = r; }
+    if (p != c) {
+      for (int q = 0; q < 3; ++q) { double t = lu[c][q]; lu[c][q] = lu[p][q]; lu[p][q] = t; }
+      int t = piv[c]; piv[c] = piv[p]; piv[p] = t;
+    }
+    for (int r = c + 1; r < 3; ++r) {
+      double l = lu[r][c] / lu[c][c];
+      lu[r][c] = l;
+      for (int q = c + 1; q < 3; ++q) lu[r][q] = fma(-l, lu[c][q], lu[r][q]);
+    }
+  }
+  m3 X;
+  for (int col = 0; col < 3; ++col) {
+    double y[3];
+    for (int r = 0; r < 3; ++r) {
+      double t = piv[r] == col ? 1.0 : 0.0;
+      for (int q = 0; q < r; ++q) t = fma(-lu[r][q], y[q], t);
+      y[r] = t;
+    }
+    for (int r = 2; r >= 0; --r) {
+      double t = y[r];
+      for (int q = r + 1; q < 3; ++q) t = fma(-lu[r][q], X.a[q][col], t);
+      X.a[r][col] = t / lu[r][r];
+    }
+  }
+  return X;
+}
+
 /* flux_decomposition.hpp:34-120 for direction X1 */
 static void roe_average(double *prim, double gm1, const double *wl, const double *wr) {
   double sqrtdl = sqrt(wl[IDN]), sqrtdr = sqrt(wr[IDN]);
@@ -1473,6 +1535,142 @@ static void full_correction_col(const or_config *cfg, int nc1, int is, int ie,
     for (int n = 0; n < 5; ++n) ducol[(size_t)n * nc1 + i] = delta[i].a[n];
 }
 
+/* ImplicitSolver::PartialCorrection (partial_correction.cpp:26-177): the
+ * (density, x1 momentum, energy) sub-blocks of the same |A| and flux
+ * Jacobians (3x3, rows and columns IDN, IVX, IEN), gravity in Phi, the
+ * reflecting ends diag(1, -1, 1), the generic ForwardSweep and
+ * BackwardSubstitution (forward_backward.hpp:15-126) on 3-vectors, and only
+ * du(IDN), du(IVX), du(IEN) replaced (the transverse tendencies stay
+ * explicit).  The flux correction term corr is zero in the reference. */
+static const int SUB3[3] = {IDN, IVX, IEN};
+
+static void partial_correction_col(const or_config *cfg, int nc1, int is, int ie,
+                                   const double *wcol, double *ducol, const double *a1,
+                                   const double *vol, double dt, m3 *a, m3 *b, m3 *c,
+                                   v3 *delta, m5 *dfdq) {
+  double gamma = cfg->gamma, grav = cfg->grav;
+  m3 Phi, Dt;
+  memset(&Phi, 0, sizeof(Phi));
+  memset(&Dt, 0, sizeof(Dt));
+  Phi.a[1][0] = grav;
+  Phi.a[2][1] = grav;
+  for (int q = 0; q < 3; ++q) Dt.a[q][q] = 1. / dt;
+#define WCOL(dst, ii) \
+  for (int n = 0; n < 5; ++n) (dst)[n] = wcol[(size_t)n * nc1 + (ii)]
+  double wl[5], wr[5], prim[5];
+  double gm1c = (gamma - 1.) * 1. / 1.;
+  for (int i = is - 2; i <= ie + 1; ++i) {
+    WCOL(wr, i);
+    dfdq[i] = flux_jacobian(gm1c, wr);
+  }
+  WCOL(wl, is - 2);
+  WCOL(wr, is - 1);
+  double gm1 = 0.5 * (gm1c + gm1c);
+  roe_average(prim, gm1, wl, wr);
+  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  m5 Am = abs_a(prim, cs, gm1), Ap;
+  for (int i = is - 1; i <= ie; ++i) {
+    WCOL(wl, i);
+    WCOL(wr, i + 1);
+    roe_average(prim, gm1, wl, wr);
+    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    Ap = abs_a(prim, cs, gm1);
+    double aleft = a1[i], aright = a1[i + 1], v = vol[i];
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) {
+        int R = SUB3[r], Q = SUB3[q];
+        a[i].a[r][q] = (Am.a[R][Q] * aleft + Ap.a[R][Q] * aright +
+                        (aright - aleft) * dfdq[i].a[R][Q]) /
+                           (2. * v) +
+                       Dt.a[r][q] - Phi.a[r][q];
+        b[i].a[r][q] = -(Am.a[R][Q] + dfdq[i - 1].a[R][Q]) * aleft / (2. * v);
+        c[i].a[r][q] = -(Ap.a[R][Q] - dfdq[i + 1].a[R][Q]) * aright / (2. * v);
+      }
+    Am = Ap;
+  }
+#undef WCOL
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) {
+      double sgn = q == 1 ? -1. : 1.;
+      a[is].a[r][q] += b[is].a[r][q] * sgn;
+    }
+  for (int r = 0; r < 3; ++r)
+    for (int q = 0; q < 3; ++q) {
+      double sgn = q == 1 ? -1. : 1.;
+      a[ie].a[r][q] += c[ie].a[r][q] * sgn;
+    }
+  for (int i = is; i <= ie; ++i) {
+    v3 rhs;
+    for (int n = 0; n < 3; ++n) rhs.a[n] = ducol[(size_t)SUB3[n] * nc1 + i] / dt;
+    if (i == is) {
+      a[i] = m3_inv(&a[i]);
+      delta[i] = m3_mulv(&a[i], &rhs);
+      a[i] = m3_mul(&a[i], &c[i]);
+    } else {
+      m3 ba = m3_mul(&b[i], &a[i - 1]);
+      m3 t;
+      for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) t.a[r][q] = a[i].a[r][q] - ba.a[r][q];
+      a[i] = m3_inv(&t);
+      v3 bd = m3_mulv(&b[i], &delta[i - 1]);
+      v3 tv;
+      for (int n = 0; n < 3; ++n) tv.a[n] = rhs.a[n] - bd.a[n];
+      delta[i] = m3_mulv(&a[i], &tv);
+      a[i] = m3_mul(&a[i], &c[i]);
+    }
+  }
+  for (int i = ie - 1; i >= is; --i) {
+    v3 ad = m3_mulv(&a[i], &delta[i + 1]);
+    for (int n = 0; n < 3; ++n) delta[i].a[n] = delta[i].a[n] - ad.a[n];
+  }
+  for (int i = is; i <= ie; ++i)
+    for (int n = 0; n < 3; ++n) ducol[(size_t)SUB3[n] * nc1 + i] = delta[i].a[n];
+}
+
+void or_implicit_column3(const or_config *cfg, int nx1, const double *w_col, double *du_col,
+                         const double *area1, const double *vol, double dt) {
+  int nc1 = nx1 + 2 * NG, is = NG, ie = NG + nx1 - 1;
+  m3 *a = calloc(nc1, sizeof(m3)), *b = calloc(nc1, sizeof(m3)), *c = calloc(nc1, sizeof(m3));
+  m5 *dfdq = calloc(nc1, sizeof(m5));
+  v3 *delta = calloc(nc1, sizeof(v3));
+  double *a1 = calloc(nc1 + 1, sizeof(double)), *vv = calloc(nc1, sizeof(double));
+  for (int i = 0; i <= nx1; ++i) a1[is + i] = area1[i];
+  for (int i = 0; i < nx1; ++i) vv[is + i] = vol[i];
+  partial_correction_col(cfg, nc1, is, ie, w_col, du_col, a1, vv, dt, a, b, c, delta, dfdq);
+  free(a); free(b); free(c); free(dfdq); free(delta); free(a1); free(vv);
+}
+
+void or_block_thomas3(int n, const double *a, const double *b, const double *c, const double *d,
+                      double *x) {
+  m3 *A = malloc(sizeof(m3) * n);
+  v3 *D = malloc(sizeof(v3) * n);
+  for (int i = 0; i < n; ++i) {
+    m3 ai, bi, ci;
+    v3 di;
+    memcpy(ai.a, a + 9 * i, sizeof(ai.a));
+    memcpy(bi.a, b + 9 * i, sizeof(bi.a));
+    memcpy(ci.a, c + 9 * i, sizeof(ci.a));
+    memcpy(di.a, d + 3 * i, sizeof(di.a));
+    if (i > 0) {
+      m3 ba = m3_mul(&bi, &A[i - 1]);
+      for (int r = 0; r < 3; ++r)
+        for (int q = 0; q < 3; ++q) ai.a[r][q] -= ba.a[r][q];
+      v3 bd = m3_mulv(&bi, &D[i - 1]);
+      for (int r = 0; r < 3; ++r) di.a[r] -= bd.a[r];
+    }
+    m3 inv = m3_inv(&ai);
+    D[i] = m3_mulv(&inv, &di);
+    A[i] = m3_mul(&inv, &ci);
+  }
+  for (int i = n - 2; i >= 0; --i) {
+    v3 ad = m3_mulv(&A[i], &D[i + 1]);
+    for (int r = 0; r < 3; ++r) D[i].a[r] -= ad.a[r];
+  }
+  for (int i = 0; i < n; ++i) memcpy(x + 3 * i, D[i].a, sizeof(D[i].a));
+  free(A);
+  free(D);
+}
+
 void or_implicit_column(const or_config *cfg, int nx1, const double *w_col, double *du_col,
                         const double *area1, const double *vol, double dt) {
   int nc1 = nx1 + 2 * NG, is = NG, ie = NG + nx1 - 1;
@@ -1496,6 +1694,8 @@ static void implicit_correct(or_solver *s, double dt) {
     v5 *delta = calloc(nc1, sizeof(v5));
     double *wcol = malloc(sizeof(double) * 5 * nc1), *ducol = malloc(sizeof(double) * 5 * nc1);
     double *a1 = malloc(sizeof(double) * (nc1 + 1)), *vv = malloc(sizeof(double) * nc1);
+    m3 *a3 = calloc(nc1, sizeof(m3)), *b3 = calloc(nc1, sizeof(m3)), *c3 = calloc(nc1, sizeof(m3));
+    v3 *delta3 = calloc(nc1, sizeof(v3));
 #pragma omp for collapse(2) schedule(static)
     for (int p = 0; p < 6; ++p)
       for (int k = s->ks; k <= s->ke; ++k) {
@@ -1513,13 +1713,18 @@ static void implicit_correct(or_solver *s, double dt) {
           }
           for (int i = s->is; i <= s->ie + 1; ++i) a1[i] = face1_area(s, P, k, j, i);
           for (int i = s->is; i <= s->ie; ++i) vv[i] = cell_volume(s, P, k, j, i);
-          full_correction_col(&s->cfg, nc1, s->is, s->ie, wcol, ducol, a1, vv, dt, a, b, c,
-                              delta, dfdq);
+          if (s->cfg.implicit_flag & 8)
+            full_correction_col(&s->cfg, nc1, s->is, s->ie, wcol, ducol, a1, vv, dt, a, b, c,
+                                delta, dfdq);
+          else
+            partial_correction_col(&s->cfg, nc1, s->is, s->ie, wcol, ducol, a1, vv, dt, a3, b3,
+                                   c3, delta3, dfdq);
           for (int n = 0; n < 5; ++n)
             for (int i = s->is; i <= s->ie; ++i) P->du[C4(n, k, j, i)] = ducol[(size_t)n * nc1 + i];
         }
       }
     free(a); free(b); free(c); free(dfdq); free(delta); free(wcol); free(ducol); free(a1); free(vv);
+    free(a3); free(b3); free(c3); free(delta3);
   }
 }
 
@@ -1700,7 +1905,7 @@ static void stage_update(or_solver *s, int st, double wght) {
  * the column solve on du when enabled, then u += du */
 static void stage_implicit(or_solver *s, double wght) {
   if (s->cfg.eos != OR_EOS_IDEAL) return;
-  int solve = (s->cfg.implicit_flag & 1) && (s->cfg.implicit_flag & 8);
+  int solve = s->cfg.implicit_flag & 1;  /* FullCorrection with bit 8, else PartialCorrection */
   if (solve) implicit_correct(s, wght);
   for (int p = 0; p < 6; ++p) {
     or_panel *P = &s->pb[p];

@@ -13,7 +13,9 @@
  *   pinned by golden vectors : or_weno5 / or_weno3 (tests/test_weno.cpp:11-29)
  *   pinned by closed forms   : geometry (areas/volumes), vector transforms,
  *                              ghost remap of constant / smooth fields,
- *                              block-Thomas vs dense solve, Riemann consistency,
+ *                              block-Thomas vs dense solve (5x5 and 3x3), the
+                              3x3 partial correction against the decoupled 5x5
+                              one, Riemann consistency,
  *                              W92 user outputs (known answers, rigid lat-lon
  *                              map, wind speed and direction), the redo path
  *                              (plain stepping without failures; a redone step
@@ -39,7 +41,7 @@ typedef struct or_config {
   int nx1;            /* radial cells */
   int eos;            /* OR_EOS_* */
   int rsolver;        /* OR_RS_* */
-  int implicit_flag;  /* 0, 1 (3x3 partial: unsupported), 9 (5x5 full, x1) */
+  int implicit_flag;  /* bit 1: x1 column solve, bit 8: 5x5 full, else 3x3 partial */
   int ntracer;        /* passive scalars */
   int coriolis;       /* OR_COR_* */
   int cfl_x1;         /* 1: x1 widths enter the CFL minimum */
@@ -128,6 +130,11 @@ void or_flux_jacobian(double gm1, const double *w, double *out25);
 void or_eigensystem(const double *prim, double cs, double gm1, double *R25, double *Ri25,
                     double *absA25);
 void or_roe_average(const double *wl, const double *wr, double gm1, double *prim);
+/* the partial (3x3: IDN, IVX, IEN) column correction and its block-Thomas */
+void or_implicit_column3(const or_config *cfg, int nx1, const double *w_col, double *du_col,
+                         const double *area1, const double *vol, double dt);
+void or_block_thomas3(int n, const double *a, const double *b, const double *c, const double *d,
+                      double *x);
 void or_block_thomas(int n, const double *a, const double *b, const double *c,
                      const double *rhs, double *x);
 void or_vec_maps(double X, double Y, int blockID, double V2, double V3, double *U, double *V,

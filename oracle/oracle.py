@@ -82,6 +82,9 @@ def lib():
             "or_eigensystem": (None, [_DP, C.c_double, C.c_double, _DP, _DP, _DP]),
             "or_roe_average": (None, [_DP, _DP, C.c_double, _DP]),
             "or_block_thomas": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
+            "or_implicit_column3": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
+                                           C.c_double]),
+            "or_block_thomas3": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
             "or_vec_maps": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                    C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -92,3 +92,19 @@ def test_invalid_configs_are_rejected_without_a_gpu(ex):
         for k, v in bad.items():
             setattr(cfg, k, v)
         assert L.exo_describe(C.byref(cfg), 0, None, 0, C.byref(n)) != 0, bad
+
+
+def test_implicit_flags(ex):
+    """x1 column correction: bit 1 solves, bit 8 picks the full system; the
+    x2/x3 bits are not supported."""
+    L = ex.lib()
+    n = C.c_size_t()
+    for flag, ok in ((0, True), (1, True), (8, True), (9, True), (2, False), (3, False),
+                     (13, False), (-1, False)):
+        cfg = atm_config(ex, nx=8, nx1=8, implicit=flag)
+        rc = L.exo_describe(C.byref(cfg), 0, None, 0, C.byref(n))
+        assert (rc == 0) == ok, flag
+    full = ex.footprint(atm_config(ex, nx=8, nx1=8, implicit=9))
+    part = ex.footprint(atm_config(ex, nx=8, nx1=8, implicit=1))
+    none = ex.footprint(atm_config(ex, nx=8, nx1=8, implicit=0))
+    assert none < part < full

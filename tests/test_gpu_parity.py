@@ -101,14 +101,24 @@ def test_shallow_water_steps(ex, orc, rsolver):
     assert e.max() <= TOL, e
 
 
-@pytest.mark.parametrize("rsolver,implicit", [(3, 9), (2, 9), (3, 0)])
+@pytest.mark.parametrize("rsolver,implicit", [(3, 9), (2, 9), (3, 0), (3, 1), (2, 1), (3, 8)])
 def test_atmosphere_steps(ex, orc, rsolver, implicit):
     cfg = atm_config(ex, nx=8, nx1=8, rsolver=rsolver, implicit=implicit)
-    nsteps = 3 if implicit else 1
+    nsteps = 3 if implicit & 1 else 1
     dg, do, ug, uo, _, _ = _run_both(ex, orc, cfg, 1, nsteps)
     assert dg == do
     e = rel_err(interior(ug, 8), interior(uo, 8))
     assert e.max() <= TOL, e
+
+
+@pytest.mark.parametrize("rsolver,nx1", [(3, 10), (2, 3), (2, 16)])
+def test_partial_correction_bit_exact(ex, orc, rsolver, nx1):
+    """implicit_flag 1: the 3x3 (density, x1 momentum, energy) column solve of
+    partial_correction.cpp, stepped bit for bit against the oracle."""
+    cfg = atm_config(ex, nx=8, nx1=nx1, rsolver=rsolver, implicit=1)
+    dg, do, ug, uo, _, _ = _run_both(ex, orc, cfg, 1, 3)
+    assert dg == do
+    assert np.array_equal(interior(ug, nx1), interior(uo, nx1))
 
 
 def test_atmosphere_fluxes_x1(ex, orc):

@@ -25,7 +25,8 @@ def _atm(ex, rs, impl, cfl, ntracer=0):
     return cfg
 
 
-@pytest.mark.parametrize("rs,impl,cfl,nsteps", [(3, 9, 1.5, 3), (2, 0, 4.0, 4), (3, 9, 0.4, 3)])
+@pytest.mark.parametrize("rs,impl,cfl,nsteps", [(3, 9, 1.5, 3), (2, 0, 4.0, 4), (3, 9, 0.4, 3),
+                                                   (3, 1, 1.5, 3)])
 def test_step_redo_vs_oracle(ex, orc, rs, impl, cfl, nsteps):
     cfg = _atm(ex, rs, impl, cfl)
     u0, _ = ex.init_state(cfg, 1, SEED, ())

@@ -457,3 +457,76 @@ def test_redo_path(orc, ex):
     a.upload(u0)
     with pytest.raises(RuntimeError, match="Too many"):
         a.step_redo(4, 2)
+
+
+# --- partial (3x3) column correction: partial_correction.cpp:26-177 -----------
+def test_block_thomas3_vs_dense_solve(orc):
+    L = orc.lib()
+    rng = np.random.default_rng(15)
+    for n in (1, 2, 6):
+        a = rng.normal(size=(n, 3, 3)) + 6 * np.eye(3)
+        b = rng.normal(size=(n, 3, 3))
+        c = rng.normal(size=(n, 3, 3))
+        rhs = rng.normal(size=(n, 3))
+        x = np.zeros((n, 3))
+        L.or_block_thomas3(n, a.copy(), b.copy(), c.copy(), rhs.copy(), x)
+        M = np.zeros((3 * n, 3 * n))
+        for i in range(n):
+            M[3 * i:3 * i + 3, 3 * i:3 * i + 3] = a[i]
+            if i > 0:
+                M[3 * i:3 * i + 3, 3 * i - 3:3 * i] = b[i]
+            if i < n - 1:
+                M[3 * i:3 * i + 3, 3 * i + 3:3 * i + 6] = c[i]
+        ref = np.linalg.solve(M, rhs.reshape(-1))
+        assert np.allclose(x.reshape(-1), ref, rtol=1e-11, atol=1e-12)
+
+
+def _column(rng, nc1, transverse):
+    z = (np.arange(nc1) - 3 + 0.5) * 3750.0
+    w = np.zeros((5, nc1))
+    w[0] = 1.2 * np.exp(-z / 7300.0)
+    w[4] = 1e5 * np.exp(-z / 7300.0)
+    w[1] = rng.normal(scale=2.0, size=nc1)
+    if transverse:
+        w[2:4] = rng.normal(scale=2.0, size=(2, nc1))
+    return w
+
+
+def test_partial_column_is_the_decoupled_full_column(orc, ex):
+    """With no transverse velocity and no transverse tendency the 5x5 system
+    decouples, and its (density, x1 momentum, energy) part is the partial
+    correction's system: the two solvers agree there.  The partial solver also
+    leaves the transverse tendencies untouched, returns du as dt -> 0, maps zero
+    to zero and is linear."""
+    cfg = orc.or_config(atm_config(ex, nx1=8))
+    nx1, nc1 = 8, 14
+    rng = np.random.default_rng(9)
+    area = np.full(nx1 + 1, 1.0e10)
+    vol = np.full(nx1, 3.75e13)
+    L = orc.lib()
+    w = _column(rng, nc1, transverse=False)
+    du = rng.normal(size=(5, nc1))
+    du[2:4] = 0.0
+    full, part = du.copy(), du.copy()
+    L.or_implicit_column(C.byref(cfg), nx1, w.copy(), full, area, vol, 50.0)
+    L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), part, area, vol, 50.0)
+    I = slice(3, 3 + nx1)
+    for n in (0, 1, 4):
+        assert np.allclose(part[n, I], full[n, I], rtol=1e-11, atol=1e-14), n
+    assert not np.allclose(part[:, I], du[:, I])          # the correction did something
+    w = _column(rng, nc1, transverse=True)
+    du = rng.normal(size=(5, nc1))
+    d = du.copy()
+    L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), d, area, vol, 50.0)
+    assert np.array_equal(d[2:4], du[2:4])
+    for dt, tol in ((1e-9, 1e-6), (1e-12, 1e-9)):
+        d = du.copy()
+        L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), d, area, vol, dt)
+        assert np.allclose(d[:, I], du[:, I], rtol=tol, atol=tol)
+    d0 = np.zeros_like(du)
+    L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), d0, area, vol, 50.0)
+    assert np.all(d0[:, I] == 0)
+    d1, d2 = du.copy(), 3.0 * du
+    L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), d1, area, vol, 50.0)
+    L.or_implicit_column3(C.byref(cfg), nx1, w.copy(), d2, area, vol, 50.0)
+    assert np.allclose(d2[:, I], 3.0 * d1[:, I], rtol=1e-12, atol=1e-12)
