@@ -180,3 +180,31 @@ def test_stage_by_entry_points(ex, orc, which):
     assert np.array_equal(interior(f.download()[0], nx1), interior(ug, nx1))
     for h in (g, o, f):
         h.close()
+
+
+@pytest.mark.parametrize("ntracer", [0, 2])
+def test_mixed_stage_by_entry_points(ex, ntracer):
+    """The mixed form: exo_stage (whose column solve takes the stage update in)
+    equals the single phases exo_calc_fluxes, exo_add_flux_divergence,
+    exo_implicit_correct, exo_fill_ghosts bit for bit."""
+    cfg = atm_config(ex, nx=8, nx1=8, rsolver=3, coriolis=2, ntracer=ntracer)
+    cfg.precision = ex.PREC_MIXED
+    u0, _ = ex.init_state(cfg, 1, SEED, ())
+    s0 = None
+    if ntracer:
+        s0 = np.random.default_rng(4).uniform(0, 1, (6, 2) + u0.shape[2:]) * u0[:, :1]
+    g, f = ex.Solver(cfg), ex.Solver(cfg)
+    g.upload(u0, s0)
+    f.upload(u0, s0)
+    dt = g.new_dt()
+    g.calc_fluxes()
+    g.add_flux_divergence(dt)
+    g.implicit_correct(dt)
+    g.fill_ghosts()
+    f.stage(1, dt)
+    (ug, sg), (uf, sf) = g.download(), f.download()
+    assert np.array_equal(ug, uf)
+    if ntracer:
+        assert np.array_equal(sg, sf)
+    g.close()
+    f.close()
