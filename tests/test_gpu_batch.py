@@ -17,7 +17,8 @@ SEED = 20261202
 
 
 def member_specs(ex):
-    """uneven sizes, both EOS, all solvers, tracers, both precisions"""
+    """uneven sizes, both EOS, all solvers, tracers, both precisions, the full
+    and the partial column correction"""
     mix = ex.PREC_MIXED
     return [
         (sw_config(ex, nx=8, rsolver=0), 0),
@@ -29,6 +30,8 @@ def member_specs(ex):
         (atm_config(ex, nx=10, nx1=6, rsolver=3, implicit=0), 1),
         (sw_config(ex, nx=8, rsolver=1, nx1=1), 0, mix),
         (atm_config(ex, nx=8, nx1=10, rsolver=2), 1, mix),
+        (atm_config(ex, nx=6, nx1=8, rsolver=3, implicit=1), 1),
+        (atm_config(ex, nx=8, nx1=6, rsolver=3, implicit=1), 1, mix),
     ]
 
 
@@ -130,7 +133,7 @@ def test_conserved_sums_match_exact_sum(ex, which):
     assert np.all(np.abs(s - tot) <= 4 * np.finfo(float).eps * mag), (s, tot)
 
 
-@pytest.mark.parametrize("which", [1, 2, 4, 5, 7, 8])
+@pytest.mark.parametrize("which", [1, 2, 4, 5, 7, 8, 9, 10])
 def test_mass_is_conserved(ex, which):
     spec = member_specs(ex)[which]
     g = make(ex, spec, SEED)

@@ -136,7 +136,7 @@ def test_three_or_more_tracers(ex, orc, which, nt):
     o.close()
 
 
-@pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers"])
+@pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers", "atm_partial"])
 def test_stage_by_entry_points(ex, orc, which):
     """One stage through the boundary's single phases (exo_calc_fluxes,
     exo_add_flux_divergence, exo_implicit_correct, exo_fill_ghosts), each
@@ -145,7 +145,8 @@ def test_stage_by_entry_points(ex, orc, which):
         cfg, kind, nx1 = sw_config(ex, nx=8, rsolver=1), 0, 3
     else:
         cfg, kind, nx1 = atm_config(ex, nx=6, nx1=8, rsolver=3, coriolis=2,
-                                    ntracer=2 if which == "atm_tracers" else 0), 1, 8
+                                    ntracer=2 if which == "atm_tracers" else 0,
+                                    implicit=1 if which == "atm_partial" else 9), 1, 8
     u0, _ = ex.init_state(cfg, kind, SEED, ())
     s0 = None
     if cfg.ntracer:
