@@ -37,9 +37,9 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
-def workload_config(ex, precision):
+def workload_config(ex, precision, implicit_flag=9):
     return ex.make_config(nx=96, nx1=64, eos=ex.EOS_IDEAL, rsolver=ex.RS_LMARS,
-                          implicit_flag=9, ntracer=0, coriolis=ex.COR_NONE, cfl_x1=1,
+                          implicit_flag=implicit_flag, ntracer=0, coriolis=ex.COR_NONE, cfl_x1=1,
                           x1min=6371000.0, x1max=6371000.0 + 30000.0, gamma=1.4,
                           grav=-9.81, omega=7.292e-5, cfl=0.4, precision=precision)
 
@@ -109,11 +109,15 @@ def rel_l2(a, b):
     I = (slice(None), slice(None), slice(3, -3), slice(3, -3), slice(3, -3))
     d = (a[I] - b[I]).reshape(a.shape[0], a.shape[1], -1)
     r = b[I].reshape(d.shape)
-    return np.sqrt((d ** 2).sum(axis=(0, 2)) / (r ** 2).sum(axis=(0, 2)))
+    num, den = (d ** 2).sum(axis=(0, 2)), (r ** 2).sum(axis=(0, 2))
+    # a variable that is zero in b (the shallow-water x1 momentum): 0 if a
+    # matches it, else inf
+    return np.where(den > 0, np.sqrt(num / np.where(den > 0, den, 1.0)),
+                    np.where(num > 0, np.inf, 0.0))
 
 
-def run_gpu(ex, precision, steps, warmup, perturb=0.0):
-    cfg = workload_config(ex, precision)
+def run_gpu(ex, precision, steps, warmup, perturb=0.0, implicit_flag=9):
+    cfg = workload_config(ex, precision, implicit_flag)
     cells = 6 * cfg.nx * cfg.nx * cfg.nx1
     u0, s0 = ex.init_state(cfg, 1, SEED, IC_PARAMS)
     if perturb:  # relative perturbation of the initial state (sensitivity floor)
@@ -136,6 +140,24 @@ def run_gpu(ex, precision, steps, warmup, perturb=0.0):
     g.close()
     sec = t1 - t0
     return dict(cells=cells, sec=sec, dt=dt, prof=prof, finite=finite, nv=5, u=u)
+
+
+def partial_correction(ex, steps, warmup):
+    """The same workload with implicit_flag 1: the 3x3 (density, x1 momentum,
+    energy) column correction of partial_correction.cpp, both forms."""
+    out = {}
+    runs = {}
+    for name, prec in (("fp64", ex.PREC_FP64), ("mixed", ex.PREC_MIXED)):
+        r = run_gpu(ex, prec, steps, warmup, implicit_flag=1)
+        ms, n = r["prof"].get("implicit", (0.0, 0))
+        out[name] = {"value": r["cells"] * steps / r["sec"], "ms_per_step": r["sec"] * 1e3 / steps,
+                     "implicit_ms_per_launch": ms / n if n else None, "state_finite": r["finite"]}
+        runs[name] = r["u"]
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    out["mixed_rel_l2_vs_fp64_max"] = float(rel_l2(runs["mixed"], runs["fp64"]).max())
+    out["workload"] = ("the headline workload with implicit_flag 1 (3x3 partial correction), "
+                       f"{steps} steps")
+    return out
 
 
 def cpu_model():
@@ -448,6 +470,9 @@ def main():
     ap.add_argument("--only-ensemble", action="store_true",
                     help="only the 256-instance shallow-water batch, both forms (for profiling)")
     ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--implicit-flag", type=int, default=9, choices=(1, 9),
+                    help="column correction of the headline workload: 9 full 5x5 (the "
+                         "benchmark), 1 partial 3x3 (for profiling k_implicit3)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc pass)")
@@ -472,15 +497,17 @@ def main():
         emx, _ = ensemble(ex, ex.PREC_MIXED, args.ensemble_steps)
         print(json.dumps({"ensemble": {"fp64": e64, "mixed": emx}}))
         return
-    r64 = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup)
-    rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup)
+    iflag = args.implicit_flag
+    r64 = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, implicit_flag=iflag)
+    rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup, implicit_flag=iflag)
     # the same seeded run in both forms: accuracy of the mixed form vs FP64
     l2 = rel_l2(rmx["u"], r64["u"])
     linf = rel_linf(rmx["u"], r64["u"])
     # what FP32 rounding of the input alone does to the FP64 result: the
     # noisy atmosphere amplifies any perturbation, so this is the floor any
     # FP32 computation sits on
-    rfl = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, perturb=2.0 ** -24)
+    rfl = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, perturb=2.0 ** -24,
+                  implicit_flag=iflag)
     floor = rel_l2(rfl["u"], r64["u"])
     del rfl
     r = rmx
@@ -496,7 +523,8 @@ def main():
     achieved = (bpc * r["cells"] / avg_s / 1e9) if bpc else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": measured_traffic(dom), "avg_launch_ms": tot_ms / cnt,
+            "traffic": measured_traffic(dom) if iflag == 9 else None,
+            "avg_launch_ms": tot_ms / cnt,
             "bytes_per_cell": bpc}
     kernels = kernel_rooflines(prof, r["cells"], r["nv"], True)
     for k in kernels:
@@ -523,6 +551,8 @@ def main():
     big = large(ex, args.large_steps) if args.large_steps > 0 and not args.only_headline else None
     outp = None if args.only_headline else output_bench(ex)
     recov = None if args.only_headline else recovery_bench(ex)
+    part = None if args.only_headline else partial_correction(ex, min(args.steps, 50),
+                                                               args.warmup)
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
@@ -538,7 +568,8 @@ def main():
         "data": "synthetic (seeded isothermal atmosphere, 1% temperature noise)",
         "config": {"workload": "exo3 3D ideal gas, 6x96x96 cells x 64 levels, WENO5+LMARS, "
                                "implicit 5x5 vertical, SSPRK3 CFL 0.4", "precision": "mixed",
-                   "cells": r["cells"], "dt_last_s": r["dt"], "state_finite": r["finite"]},
+                   "cells": r["cells"], "dt_last_s": r["dt"], "state_finite": r["finite"],
+                   "implicit_flag": iflag},
         "fp64": {"value": value64, "ms_per_step": r64["sec"] * 1e3 / args.steps,
                  "dt_last_s": r64["dt"], "kernels": kernels64},
         "mixed_over_fp64": value / value64,
@@ -557,6 +588,7 @@ def main():
         "large": big,
         "output": outp,
         "recovery": recov,
+        "partial_correction": part,
     }
     print(json.dumps(out))
 

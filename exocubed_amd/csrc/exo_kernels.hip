@@ -1281,6 +1281,8 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   auto wload = [&](int i, double *w) {
 #pragma unroll
     for (int n = 0; n < 5; ++n) w[n] = W[n * vs + c0 + (size_t)i * jp];
+  };
+  auto wproj = [&](double *w) {
     w[IVY] += w[IVZ] * cth;
     w[IVZ] *= sth;
   };
@@ -1290,7 +1292,9 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   auto erow = [&](const double *w) { return k1 * w[IPR] / w[IDN]; };
   double wm[5], w0[5], wp[5], prim[5];
   wload(is - 1, wm);
+  wproj(wm);
   wload(is, w0);
+  wproj(w0);
   roe_average(prim, gm1, wm, w0);
   double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
   M5 Am = abs_a(prim, cs, gm1);
@@ -1298,13 +1302,26 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   double ydt = 1. / dt;
   M3 Aprev;
   V3 dprev;
-  for (int i = is; i <= ie; ++i) {
-    wload(i + 1, wp);
-    double ep = erow(wp);
-    V3 rhs;
+  // the next level's state and tendency are loaded a level ahead and put to
+  // use before the level's coefficient stores
+  double wnext[5], enext;
+  V3 rnext;
+  wload(is + 1, wnext);
+  wproj(wnext);
+  enext = erow(wnext);
 #pragma unroll
-    for (int n = 0; n < 3; ++n)
-      rhs.a[n] = div_rn(DU[SUB3[n] * vs + c0 + (size_t)i * jp], dt, ydt);
+  for (int n = 0; n < 3; ++n) rnext.a[n] = div_rn(DU[SUB3[n] * vs + c0 + (size_t)is * jp], dt, ydt);
+  for (int i = is; i <= ie; ++i) {
+    double wnx[5], dun[3];
+#pragma unroll
+    for (int n = 0; n < 5; ++n) wp[n] = wnext[n];
+    double ep = enext;
+    V3 rhs = rnext;
+    if (i < ie) {
+      wload(i + 2, wnx);
+#pragma unroll
+      for (int n = 0; n < 3; ++n) dun[n] = DU[SUB3[n] * vs + c0 + (size_t)(i + 1) * jp];
+    }
     double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
     double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
     double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
@@ -1352,6 +1369,14 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
     V3 delta = m3_mulv(ainv, tv);
     Aprev = m3_mul(ainv, c);
     dprev = delta;
+    if (i < ie) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) wnext[n] = wnx[n];
+      wproj(wnext);
+      enext = erow(wnext);
+#pragma unroll
+      for (int n = 0; n < 3; ++n) rnext.a[n] = div_rn(dun[n], dt, ydt);
+    }
     double cv[12];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -1372,32 +1397,69 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   }
   // backward substitution, then u += du: the corrected (density, x1 momentum,
   // energy) and the explicit transverse momenta
-  V3 dnext = dprev;
-  for (int i = ie; i >= is; --i) {
-    V3 d = dnext;
-    if (i < ie) {
-      double cv[12];
+  // each level reads 6 coefficient pieces, 5 states and 2 tendencies; levels
+  // are loaded four ahead (52 loads in flight, within the 63 the vector-memory
+  // counter tracks)
+  auto ustore = [&](int i, const V3 &d, const double *uo) {
+    size_t ci = c0 + (size_t)i * jp;
 #pragma unroll
-      for (int q2 = 0; q2 < 6; ++q2) {
-        dpair t = *cfp(i, q2);
-        cv[2 * q2] = t.x;
-        cv[2 * q2 + 1] = t.y;
-      }
+    for (int n = 0; n < 3; ++n) U[SUB3[n] * vs + ci] = uo[n] + d.a[n];
+    U[IVY * vs + ci] = uo[3] + uo[5];
+    U[IVZ * vs + ci] = uo[4] + uo[6];
+  };
+  // uo: u of the three corrected rows, u(IVY), u(IVZ), du(IVY), du(IVZ)
+  auto uload = [&](int i, double *uo) {
+    size_t ci = c0 + (size_t)i * jp;
+#pragma unroll
+    for (int n = 0; n < 3; ++n) uo[n] = U[SUB3[n] * vs + ci];
+    uo[3] = U[IVY * vs + ci];
+    uo[4] = U[IVZ * vs + ci];
+    uo[5] = DU[IVY * vs + ci];
+    uo[6] = DU[IVZ * vs + ci];
+  };
+  constexpr int PD = 4;
+  double cq[PD][12], uq[PD][7];
+  auto bload = [&](int l, double *cv, double *uo) {
+#pragma unroll
+    for (int q2 = 0; q2 < 6; ++q2) {
+      dpair t = *cfp(l, q2);
+      cv[2 * q2] = t.x;
+      cv[2 * q2 + 1] = t.y;
+    }
+    uload(l, uo);
+  };
+  V3 dnext = dprev;
+  {
+    double uo[7];
+    uload(ie, uo);
+    ustore(ie, dnext, uo);
+  }
+#pragma unroll
+  for (int q = 0; q < PD; ++q)
+    if (ie - 1 - q >= is) bload(ie - 1 - q, cq[q], uq[q]);
+  for (int i0 = ie - 1; i0 >= is; i0 -= PD) {
+#pragma unroll
+    for (int q = 0; q < PD; ++q) {
+      int i = i0 - q;
+      if (i < is) break;
+      double cv[12], uo[7];
+#pragma unroll
+      for (int t = 0; t < 12; ++t) cv[t] = cq[q][t];
+#pragma unroll
+      for (int t = 0; t < 7; ++t) uo[t] = uq[q][t];
+      if (i - PD >= is) bload(i - PD, cq[q], uq[q]);
       M3 A;
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) A.a[r][q] = cv[r * 3 + q];
+        for (int c = 0; c < 3; ++c) A.a[r][c] = cv[r * 3 + c];
       V3 ad = m3_mulv(A, dnext);
+      V3 d;
 #pragma unroll
       for (int n = 0; n < 3; ++n) d.a[n] = cv[9 + n] - ad.a[n];
+      ustore(i, d, uo);
+      dnext = d;
     }
-    size_t ci = c0 + (size_t)i * jp;
-#pragma unroll
-    for (int n = 0; n < 3; ++n) U[SUB3[n] * vs + ci] += d.a[n];
-    U[IVY * vs + ci] += DU[IVY * vs + ci];
-    U[IVZ * vs + ci] += DU[IVZ * vs + ci];
-    dnext = d;
   }
 }
 
