@@ -112,9 +112,11 @@ EXO_API int exo_add_flux_divergence(exo_handle *h, double wght);
  * 371-617) of the conserved state into the face ghost zones, then
  * cons->prim (EquationOfState::ConservedToPrimitive) and the reflecting x1 BC. */
 EXO_API int exo_fill_ghosts(exo_handle *h);
-/* ImplicitSolver::SolveImplicit3D (solve_implicit_3d.cpp:27-180) with
- * FullCorrection on the x1 columns, applied to du, then u += du
- * (ImplicitHydroTasks::ImplicitCorrection, implicit_hydro_tasks.cpp:245-266). */
+/* ImplicitSolver::SolveImplicit3D (solve_implicit_3d.cpp:27-180) on the x1
+ * columns, FullCorrection (implicit_flag 9, full_correction.cpp) or
+ * PartialCorrection (implicit_flag 1, partial_correction.cpp:26-177), applied
+ * to du, then u += du (ImplicitHydroTasks::ImplicitCorrection,
+ * implicit_hydro_tasks.cpp:245-266). */
 EXO_API int exo_implicit_correct(exo_handle *h, double dt_stage);
 /* Hydro::NewBlockTimeStep (patches/21.new_blockdt.patch:16-37) and
  * Mesh::NewTimeStep's factor-2 growth limit; returns the next dt. */

@@ -131,7 +131,8 @@ class Hydro {
 class ImplicitSolver {
  public:
   explicit ImplicitSolver(Instance &in) : h_(in.handle()) {}
-  // the vertical 5x5 block-tridiagonal correction of du, then u += du
+  // the vertical block-tridiagonal correction of du (5x5 with implicit_flag 9,
+  // 3x3 with 1), then u += du
   void SolveImplicit3D(Real dt_stage) {
     check(exo_implicit_correct(h_, dt_stage), h_, "ImplicitSolver::SolveImplicit3D");
   }
