@@ -12,8 +12,8 @@
 //   HLLC:   t_l = P + t_l', cp = P + cp', sl + sr + sm = 1
 // Shallow water instead takes each face about its left cell's FP64 state
 // (roe_sw_delta, lmars_sw_delta below).
-// Unlike the FP64 path, the FP32 WENO5 uses FMAs: the mixed form is judged by
-// its error bound, not by reproducing the reference's roundings.
+// Unlike the FP64 path, the FP32 WENO5 uses FMAs: the mixed form is held to
+// its error bound, not to the reference's roundings.
 #pragma once
 
 #include "exo_device.hpp"
