@@ -27,4 +27,4 @@ def test_host_checks():
 @pytest.mark.gpu
 def test_all_checks_on_device():
     out = _run("all")
-    assert "FAILED" not in out and "7 tests, 0 failed checks" in out
+    assert "FAILED" not in out and "8 tests, 0 failed checks" in out
