@@ -56,9 +56,13 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
         "flux_x1": nv * 8 + nv * f + (8 if mixed else 0) + nt * (8 + f),
         "flux_x2": nv * 8 + nv * f + nt * (8 + f),
         "flux_x3": nv * 8 + nv * f + nt * (8 + f),
-        # read u, u1, w, F1, F2, F3 (+ psf); write u, u1, du; scalars s, s1, G1..3
-        "update": 6 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f),
-        "implicit": 4 * nv * 8,     # read du, w, u; write u
+        # within a stage with the full column solve: read w, F1, F2, F3 (+ psf),
+        # write du; scalars s, s1, G1..3 (the weighted average of u is formed
+        # in the column solve)
+        "update": 2 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f),
+        # read du, w, u, u1 (stages 2-3) and write u, or read du, w, u and
+        # write u, u1 (stage 1)
+        "implicit": 5 * nv * 8,
         "prim": 2 * nv * 8 + 2 * nt * 8,  # read u (s), write w (r)
         "dt": nv * 8,               # read w
     }.get(kernel)

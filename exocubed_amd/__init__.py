@@ -25,7 +25,8 @@ KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "upd
 # launch kinds of the persistent-width tables (exo_tune / exo_batch_tune)
 LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
                 "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one", "dt",
-                "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost", "implicit3")
+                "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost", "implicit3",
+                "update_na", "update_mix_na", "implicit_avg", "update_rs", "update_mix_rs")
 
 
 class Config(C.Structure):

@@ -780,7 +780,10 @@ struct StageW {
 // result: the tendency (ideal gas, whose state u is the weighted average and
 // is written here) or the new state (shallow water).  Passive scalars are
 // updated in place.
-template <bool MIX>
+// NOAVG (ideal gas within a stage whose column solve follows): the weighted
+// average of u is left to the column solve's backward substitution, which
+// reads and writes u anyway; here only the tendency is formed.
+template <bool MIX, bool NOAVG = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out) {
   double dt = st.dtv[0];
@@ -805,13 +808,16 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   for (int n = 0; n < 5; ++n) {
     if (g.skipv(n)) continue;
     size_t c = g.c(n, p, k, i, j);
-    double u = st.u[c], u1;
-    if (sw.first) {
+    double u = 0., u1;
+    if (NOAVG) {
+    } else if (sw.first) {
+      u = st.u[c];
       u1 = 0.;
       if (sw.delta != 0.0) u1 += sw.delta * u;
       u = u1;
       st.u1[c] = u1;
     } else {
+      u = st.u[c];
       u1 = st.u1[c];
       if (sw.delta != 0.0) {
         u1 += sw.delta * u;
@@ -845,7 +851,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
                  (a3r * g.T(T_PRZ3, p, k + 1, j) - a3l * g.T(T_PRZ3, p, k, j))) * P;
     }
     if (tend) {
-      if (!keep_u) st.u[c] = u;
+      if (!keep_u && !NOAVG) st.u[c] = u;
       out[n] = 0.;
       out[n] -= div_rn(wght * dflx, vol, yvol);
     } else {
@@ -945,7 +951,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   }
 }
 
-template <bool MIX>
+template <bool MIX, bool NOAVG = false>
 __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   // the validity flag describes the primitive conversion that follows this
@@ -961,7 +967,7 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   r /= ni;
   int k = g.ks + (int)(r % N), p = (int)(r / N);
   double out[5];
-  update_cell<MIX>(g, st, p, k, i, j, sw, out);
+  update_cell<MIX, NOAVG>(g, st, p, k, i, j, sw, out);
   ADDR(1) double *dst = g.eos == EOS_IDEAL ? st.du : st.u;
   for (int n = 0; n < g.nv; ++n)
     if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
@@ -970,11 +976,11 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   if (sw.prim && g.eos == EOS_SW) prim_from(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor);
 }
 
-template <bool MIX>
+template <bool MIX, bool NOAVG = false>
 __global__ void __launch_bounds__(BLK) k_update(const Inst *__restrict__ I_, Work w, StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    update_body<MIX>(g, st, lb, sw);
+    update_body<MIX, NOAVG>(g, st, lb, sw);
   }
 }
 
@@ -1003,8 +1009,14 @@ constexpr int IMPL_MAXNC1 = 512;  // radial cells incl. ghosts the column solve 
 // geo: LDS for the instance's x1f | x1v | dx1f, so the per-level geometry is
 // read without waiting on the vector-memory counter (which also counts the
 // coefficient stores and the prefetches in flight)
-__device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, int lb, double beta,
-                                              double *geo) {
+// AVG (within a stage): the update kernel formed only the tendency, and the
+// stage's weighted average of u (with the stage register u1) is formed here,
+// as update_cell forms it, before u += delta: u is read and written once per
+// stage instead of twice.
+template <bool AVG>
+__device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, int lb,
+                                              const StageW &sw, double *geo) {
+  double beta = sw.beta;
   for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
     geo[q] = g.x1f[q];
     if (q < g.nc1) {
@@ -1178,17 +1190,54 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   }
   // backward substitution; delta(ie) is final.  Only u += delta is stored:
   // nothing reads the correction itself afterwards.
-  V5 dnext = dprev;
+  ADDR(1) double *U1 = st.u1;
+  bool first = sw.first, keep_u = sw.first && sw.delta == 1.0;
+  auto uload = [&](int l, V5 &u, V5 &u1) {
 #pragma unroll
-  for (int n = 0; n < 5; ++n) {
-    size_t cc = n * vs + c0 + (size_t)ie * jp;
-    U[cc] += dnext.a[n];
+    for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
+    if (AVG && !first) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) u1.a[n] = U1[n * vs + c0 + (size_t)l * jp];
+    }
+  };
+  // the state delta is added to: u itself, or the stage's weighted average
+  auto ubase = [&](int l, const V5 &uo, const V5 &u1o) -> V5 {
+    if (!AVG) return uo;
+    V5 ub;
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      size_t c = n * vs + c0 + (size_t)l * jp;
+      double u = uo.a[n], u1;
+      if (first) {
+        u1 = 0.;
+        if (sw.delta != 0.0) u1 += sw.delta * u;
+        U1[c] = u1;
+        ub.a[n] = keep_u ? u : u1;
+      } else {
+        u1 = u1o.a[n];
+        if (sw.delta != 0.0) {
+          u1 += sw.delta * u;
+          U1[c] = u1;
+        }
+        ub.a[n] = sw.g1 * u + sw.g2 * u1;
+      }
+    }
+    return ub;
+  };
+  V5 dnext = dprev;
+  {
+    V5 uo, u1o;
+    uload(ie, uo, u1o);
+    V5 ub = ubase(ie, uo, u1o);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)ie * jp] = ub.a[n] + dnext.a[n];
   }
-  // levels are loaded three ahead (a ring of three: 60 loads in flight at most)
-  constexpr int PD = 3;
+  // levels are loaded ahead within the 63 loads the vector-memory counter
+  // tracks: three levels of 20 loads, or two of 25 with the stage register
+  constexpr int PD = AVG ? 2 : 3;
   M5 Aq[PD];
-  V5 dq[PD], uq[PD];
-  auto bload = [&](int l, M5 &A, V5 &d, V5 &u) {
+  V5 dq[PD], uq[PD], u1q[PD];
+  auto bload = [&](int l, M5 &A, V5 &d, V5 &u, V5 &u1) {
     double cv[30];
 #pragma unroll
     for (int q2 = 0; q2 < 15; ++q2) {
@@ -1202,35 +1251,36 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
       for (int q = 0; q < 5; ++q) A.a[r][q] = cv[r * 5 + q];
 #pragma unroll
     for (int n = 0; n < 5; ++n) d.a[n] = cv[25 + n];
-#pragma unroll
-    for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
+    uload(l, u, u1);
   };
 #pragma unroll
   for (int q = 0; q < PD; ++q)
-    if (ie - 1 - q >= is) bload(ie - 1 - q, Aq[q], dq[q], uq[q]);
+    if (ie - 1 - q >= is) bload(ie - 1 - q, Aq[q], dq[q], uq[q], u1q[q]);
   for (int i0 = ie - 1; i0 >= is; i0 -= PD) {
 #pragma unroll
     for (int q = 0; q < PD; ++q) {
       int i = i0 - q;
       if (i < is) break;
       M5 A = Aq[q];
-      V5 d = dq[q], uo = uq[q];
-      if (i - PD >= is) bload(i - PD, Aq[q], dq[q], uq[q]);
+      V5 d = dq[q], uo = uq[q], u1o = u1q[q];
+      if (i - PD >= is) bload(i - PD, Aq[q], dq[q], uq[q], u1q[q]);
       V5 ad = m5_mulv(A, dnext);
 #pragma unroll
       for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
+      V5 ub = ubase(i, uo, u1o);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = uo.a[n] + d.a[n];
+      for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = ub.a[n] + d.a[n];
       dnext = d;
     }
   }
 }
 
-__global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Work w, double beta) {
+template <bool AVG>
+__global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Work w, StageW sw) {
   __shared__ double geo[3 * (IMPL_MAXNC1 + 1)];
   WORK_LOOP(w) {
     WORK_INST(w);
-    implicit_body(g, st, lb, beta, geo);
+    implicit_body<AVG>(g, st, lb, sw, geo);
     __syncthreads();  // geo is refilled by the next block of the loop
   }
 }
@@ -2207,7 +2257,11 @@ enum {
   // strips for shallow water (its stage update writes the interior)
   K_PRIM_REST, K_PRIM_GHOST,
   // the partial (3x3) column correction
-  K_IMPL3, K_NKIND
+  K_IMPL3,
+  // within a stage, instances whose full column solve forms the stage's
+  // weighted average: their update (tendency only) and their solve; the
+  // update of every other instance
+  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_UPD_MIX_RS, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2310,6 +2364,8 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
   size_t cells = (size_t)6 * g.N * g.N * g.nx1;
   size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
   size_t cols = (size_t)g.N * g.N;
+  // the full 5x5 column solve (implicit_flag bits 1 and 8)
+  bool full = ideal && (g.implicit & 9) == 9;
   switch (kind) {
     case K_PRIM: return (int)grid1((size_t)6 * g.nc3 * g.nx1 * g.nc2);
     case K_PRIM_REST: return ideal ? (int)grid1((size_t)6 * g.nc3 * g.nx1 * g.nc2) : 0;
@@ -2328,6 +2384,11 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
+    case K_UPD_NA: return ideal && !mixed && full ? (int)grid1(cells) : 0;
+    case K_UPD_MIX_NA: return ideal && mixed && full ? (int)grid1(cells) : 0;
+    case K_IMPL_AVG: return full ? (int)grid1(cols, 64) * 6 : 0;
+    case K_UPD_RS: return (!mixed || !ideal) && !full ? (int)grid1(cells) : 0;
+    case K_UPD_MIX_RS: return mixed && ideal && !full ? (int)grid1(cells) : 0;
     // implicit_flag bit 1 solves the x1 columns, bit 8 with the full 5x5
     // system (solve_implicit_3d.cpp:102-135); without bit 1, u += du
     case K_IMPL: return ideal && (g.implicit & 1) && (g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
@@ -2604,8 +2665,26 @@ void launch_update(Launcher &L, int stage, int prim) {
   HIPCHK(hipGetLastError());
 }
 
+// the stage's update and column correction; the instances with the full
+// column solve leave the weighted average of u to it
+void launch_update_implicit(Launcher &L, int stage) {
+  int st0 = stage - 1;
+  StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
+            1, DFLOOR, SFLOOR};
+  launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
+  launch(L, K_UPD_MIX_RS, KID_UPDATE, k_update<true>, BLK, 0, sw);
+  launch(L, K_UPD_NA, KID_UPDATE, k_update<false, true>, BLK, 0, sw);
+  launch(L, K_UPD_MIX_NA, KID_UPDATE, k_update<true, true>, BLK, 0, sw);
+  launch(L, K_IMPL_AVG, KID_IMPLICIT, k_implicit<true>, 64, 0, sw);
+  launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3, 64, 0, sw.beta);
+  launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
+  HIPCHK(hipGetLastError());
+}
+
 void launch_implicit(Launcher &L, double beta) {
-  launch(L, K_IMPL, KID_IMPLICIT, k_implicit, 64, 0, beta);
+  StageW sw{};
+  sw.beta = beta;
+  launch(L, K_IMPL, KID_IMPLICIT, k_implicit<false>, 64, 0, sw);
   launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3, 64, 0, beta);
   launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
   HIPCHK(hipGetLastError());
@@ -2632,8 +2711,7 @@ void launch_fill_ghosts_stage(Launcher &L) {
 
 void run_stage(Launcher &L, int stage) {
   launch_fluxes(L);
-  launch_update(L, stage, 1);
-  launch_implicit(L, SS_BETA[stage - 1]);
+  launch_update_implicit(L, stage);
   launch_fill_ghosts_stage(L);
 }
 
