@@ -69,7 +69,8 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
 
 
 # kernel of the live profile -> its name in the rocprofv3 PMC summary
-PMC_NAMES = {"implicit": "k_implicit", "update": "k_update<true>", "flux_x1": "k_flux_x1_mix",
+PMC_NAMES = {"implicit": "k_implicit<true>", "update": "k_update<true, true>",
+             "flux_x1": "k_flux_x1_mix",
              "flux_x2": "k_flux_h_mix<1, false>", "flux_x3": "k_flux_h_mix<2, false>",
              "prim": "k_prim"}
 
