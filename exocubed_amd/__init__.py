@@ -26,7 +26,8 @@ KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "upd
 LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
                 "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one", "dt",
                 "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost", "implicit3",
-                "update_na", "update_mix_na", "implicit_avg", "update_rs", "update_mix_rs")
+                "update_na", "update_mix_na", "implicit_avg", "update_rs", "implicit3_avg",
+                "add_du_avg")
 
 
 class Config(C.Structure):

@@ -780,6 +780,25 @@ struct StageW {
 // result: the tendency (ideal gas, whose state u is the weighted average and
 // is written here) or the new state (shallow water).  Passive scalars are
 // updated in place.
+// The stage's weighted average of one value of u, formed as update_cell
+// forms it (the first stage sets the stage register u1 = delta u and keeps u
+// when delta is 1; later stages update u1 when delta != 0); *u1p receives u1
+// where update_cell stores it.
+__device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u1,
+                                            ADDR(1) double *u1p) {
+  if (sw.first) {
+    double v = 0.;
+    if (sw.delta != 0.0) v += sw.delta * u;
+    *u1p = v;
+    return sw.delta == 1.0 ? u : v;
+  }
+  if (sw.delta != 0.0) {
+    u1 += sw.delta * u;
+    *u1p = u1;
+  }
+  return sw.g1 * u + sw.g2 * u1;
+}
+
 // NOAVG (ideal gas within a stage whose column solve follows): the weighted
 // average of u is left to the column solve's backward substitution, which
 // reads and writes u anyway; here only the tendency is formed.
@@ -1191,7 +1210,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   // backward substitution; delta(ie) is final.  Only u += delta is stored:
   // nothing reads the correction itself afterwards.
   ADDR(1) double *U1 = st.u1;
-  bool first = sw.first, keep_u = sw.first && sw.delta == 1.0;
+  bool first = sw.first;
   auto uload = [&](int l, V5 &u, V5 &u1) {
 #pragma unroll
     for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
@@ -1205,23 +1224,8 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
     if (!AVG) return uo;
     V5 ub;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) {
-      size_t c = n * vs + c0 + (size_t)l * jp;
-      double u = uo.a[n], u1;
-      if (first) {
-        u1 = 0.;
-        if (sw.delta != 0.0) u1 += sw.delta * u;
-        U1[c] = u1;
-        ub.a[n] = keep_u ? u : u1;
-      } else {
-        u1 = u1o.a[n];
-        if (sw.delta != 0.0) {
-          u1 += sw.delta * u;
-          U1[c] = u1;
-        }
-        ub.a[n] = sw.g1 * u + sw.g2 * u1;
-      }
-    }
+    for (int n = 0; n < 5; ++n)
+      ub.a[n] = stage_avg(sw, uo.a[n], first ? 0. : u1o.a[n], U1 + n * vs + c0 + (size_t)l * jp);
     return ub;
   };
   V5 dnext = dprev;
@@ -1292,8 +1296,10 @@ __global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Wo
 // the transverse momenta take their explicit tendency.  Ends with u += du.
 constexpr int SUB3[3] = {IDN, IVX, IEN};
 
+template <bool AVG>
 __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st, int lb,
-                                               double beta, double *geo) {
+                                               const StageW &sw, double *geo) {
+  double beta = sw.beta;
   for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
     geo[q] = g.x1f[q];
     if (q < g.nc1) {
@@ -1447,28 +1453,37 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   }
   // backward substitution, then u += du: the corrected (density, x1 momentum,
   // energy) and the explicit transverse momenta
-  // each level reads 6 coefficient pieces, 5 states and 2 tendencies; levels
-  // are loaded four ahead (52 loads in flight, within the 63 the vector-memory
-  // counter tracks)
+  // each level reads 6 coefficient pieces, 5 states, 2 tendencies and (AVG,
+  // after the first stage) 5 stage registers; levels are loaded four ahead
+  // (52 loads in flight), or three (54) with the stage registers, within the
+  // 63 the vector-memory counter tracks.  AVG (within a stage): delta is
+  // added to the stage's weighted average of u, formed here.
+  ADDR(1) double *U1 = st.u1;
   auto ustore = [&](int i, const V3 &d, const double *uo) {
     size_t ci = c0 + (size_t)i * jp;
+    double ub[5];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) U[SUB3[n] * vs + ci] = uo[n] + d.a[n];
-    U[IVY * vs + ci] = uo[3] + uo[5];
-    U[IVZ * vs + ci] = uo[4] + uo[6];
+    for (int n = 0; n < 5; ++n)
+      ub[n] = AVG ? stage_avg(sw, uo[n], sw.first ? 0. : uo[5 + n], U1 + n * vs + ci) : uo[n];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) U[SUB3[n] * vs + ci] = ub[SUB3[n]] + d.a[n];
+    U[IVY * vs + ci] = ub[IVY] + uo[10];
+    U[IVZ * vs + ci] = ub[IVZ] + uo[11];
   };
-  // uo: u of the three corrected rows, u(IVY), u(IVZ), du(IVY), du(IVZ)
+  // uo: u (5), the stage register u1 (5), du(IVY), du(IVZ)
   auto uload = [&](int i, double *uo) {
     size_t ci = c0 + (size_t)i * jp;
 #pragma unroll
-    for (int n = 0; n < 3; ++n) uo[n] = U[SUB3[n] * vs + ci];
-    uo[3] = U[IVY * vs + ci];
-    uo[4] = U[IVZ * vs + ci];
-    uo[5] = DU[IVY * vs + ci];
-    uo[6] = DU[IVZ * vs + ci];
+    for (int n = 0; n < 5; ++n) uo[n] = U[n * vs + ci];
+    if (AVG && !sw.first) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) uo[5 + n] = U1[n * vs + ci];
+    }
+    uo[10] = DU[IVY * vs + ci];
+    uo[11] = DU[IVZ * vs + ci];
   };
-  constexpr int PD = 4;
-  double cq[PD][12], uq[PD][7];
+  constexpr int PD = AVG ? 3 : 4;
+  double cq[PD][12], uq[PD][12];
   auto bload = [&](int l, double *cv, double *uo) {
 #pragma unroll
     for (int q2 = 0; q2 < 6; ++q2) {
@@ -1480,7 +1495,7 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   };
   V3 dnext = dprev;
   {
-    double uo[7];
+    double uo[12];
     uload(ie, uo);
     ustore(ie, dnext, uo);
   }
@@ -1492,11 +1507,11 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
     for (int q = 0; q < PD; ++q) {
       int i = i0 - q;
       if (i < is) break;
-      double cv[12], uo[7];
+      double cv[12], uo[12];
 #pragma unroll
       for (int t = 0; t < 12; ++t) cv[t] = cq[q][t];
 #pragma unroll
-      for (int t = 0; t < 7; ++t) uo[t] = uq[q][t];
+      for (int t = 0; t < 12; ++t) uo[t] = uq[q][t];
       if (i - PD >= is) bload(i - PD, cq[q], uq[q]);
       M3 A;
 #pragma unroll
@@ -1513,19 +1528,23 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   }
 }
 
+template <bool AVG>
 __global__ void __launch_bounds__(64) k_implicit3(const Inst *__restrict__ I_, Work w,
-                                                  double beta) {
+                                                  StageW sw) {
   __shared__ double geo[3 * (IMPL_MAXNC1 + 1)];
   WORK_LOOP(w) {
     WORK_INST(w);
-    implicit3_body(g, st, lb, beta, geo);
+    implicit3_body<AVG>(g, st, lb, sw, geo);
     __syncthreads();
   }
 }
 
 
 // u += du without the column solve (ImplicitHydroTasks with implicit_flag 0)
-__device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, int lb) {
+// AVG (within a stage): u = the stage's weighted average + du
+template <bool AVG>
+__device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, int lb,
+                                            const StageW &sw) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
   size_t total = (size_t)6 * N * ni * N;
@@ -1537,14 +1556,17 @@ __device__ __forceinline__ void add_du_body(const DGrid &g, const DState &st, in
   int k = g.ks + (int)(r % N), p = (int)(r / N);
   for (int n = 0; n < g.nv; ++n) {
     size_t c = g.c(n, p, k, i, j);
-    st.u[c] += st.du[c];
+    double u = st.u[c];
+    if (AVG) u = stage_avg(sw, u, sw.first ? 0. : st.u1[c], st.u1 + c);
+    st.u[c] = u + st.du[c];
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_add_du(const Inst *__restrict__ I_, Work w) {
+template <bool AVG>
+__global__ void __launch_bounds__(BLK) k_add_du(const Inst *__restrict__ I_, Work w, StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    add_du_body(g, st, lb);
+    add_du_body<AVG>(g, st, lb, sw);
   }
 }
 
@@ -2261,7 +2283,7 @@ enum {
   // within a stage, instances whose full column solve forms the stage's
   // weighted average: their update (tendency only) and their solve; the
   // update of every other instance
-  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_UPD_MIX_RS, K_NKIND
+  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2384,11 +2406,13 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
-    case K_UPD_NA: return ideal && !mixed && full ? (int)grid1(cells) : 0;
-    case K_UPD_MIX_NA: return ideal && mixed && full ? (int)grid1(cells) : 0;
+    case K_UPD_NA: return ideal && !mixed ? (int)grid1(cells) : 0;
+    case K_UPD_MIX_NA: return ideal && mixed ? (int)grid1(cells) : 0;
     case K_IMPL_AVG: return full ? (int)grid1(cols, 64) * 6 : 0;
-    case K_UPD_RS: return (!mixed || !ideal) && !full ? (int)grid1(cells) : 0;
-    case K_UPD_MIX_RS: return mixed && ideal && !full ? (int)grid1(cells) : 0;
+    case K_UPD_RS: return !ideal ? (int)grid1(cells) : 0;
+    case K_IMPL3_AVG:
+      return ideal && (g.implicit & 1) && !(g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
+    case K_ADD_DU_AVG: return ideal && !(g.implicit & 1) ? (int)grid1(cells) : 0;
     // implicit_flag bit 1 solves the x1 columns, bit 8 with the full 5x5
     // system (solve_implicit_3d.cpp:102-135); without bit 1, u += du
     case K_IMPL: return ideal && (g.implicit & 1) && (g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
@@ -2672,12 +2696,11 @@ void launch_update_implicit(Launcher &L, int stage) {
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
             1, DFLOOR, SFLOOR};
   launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
-  launch(L, K_UPD_MIX_RS, KID_UPDATE, k_update<true>, BLK, 0, sw);
   launch(L, K_UPD_NA, KID_UPDATE, k_update<false, true>, BLK, 0, sw);
   launch(L, K_UPD_MIX_NA, KID_UPDATE, k_update<true, true>, BLK, 0, sw);
   launch(L, K_IMPL_AVG, KID_IMPLICIT, k_implicit<true>, 64, 0, sw);
-  launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3, 64, 0, sw.beta);
-  launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
+  launch(L, K_IMPL3_AVG, KID_IMPLICIT, k_implicit3<true>, 64, 0, sw);
+  launch(L, K_ADD_DU_AVG, KID_IMPLICIT, k_add_du<true>, BLK, 0, sw);
   HIPCHK(hipGetLastError());
 }
 
@@ -2685,8 +2708,8 @@ void launch_implicit(Launcher &L, double beta) {
   StageW sw{};
   sw.beta = beta;
   launch(L, K_IMPL, KID_IMPLICIT, k_implicit<false>, 64, 0, sw);
-  launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3, 64, 0, beta);
-  launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du, BLK, 0);
+  launch(L, K_IMPL3, KID_IMPLICIT, k_implicit3<false>, 64, 0, sw);
+  launch(L, K_ADD_DU, KID_IMPLICIT, k_add_du<false>, BLK, 0, sw);
   HIPCHK(hipGetLastError());
 }
 
