@@ -136,17 +136,19 @@ def test_three_or_more_tracers(ex, orc, which, nt):
     o.close()
 
 
-@pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers", "atm_partial"])
+@pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers", "atm_partial", "atm_explicit"])
 def test_stage_by_entry_points(ex, orc, which):
     """One stage through the boundary's single phases (exo_calc_fluxes,
     exo_add_flux_divergence, exo_implicit_correct, exo_fill_ghosts), each
-    against the oracle's phase, and equal to the fused exo_stage."""
+    against the oracle's phase, and equal to exo_stage (whose update leaves the
+    weighted average of u to the column solve or to u += du)."""
     if which == "sw":
         cfg, kind, nx1 = sw_config(ex, nx=8, rsolver=1), 0, 3
     else:
-        cfg, kind, nx1 = atm_config(ex, nx=6, nx1=8, rsolver=3, coriolis=2,
-                                    ntracer=2 if which == "atm_tracers" else 0,
-                                    implicit=1 if which == "atm_partial" else 9), 1, 8
+        flag = {"atm_partial": 1, "atm_explicit": 0}.get(which, 9)
+        cfg = atm_config(ex, nx=6, nx1=8, rsolver=3, coriolis=2,
+                         ntracer=2 if which == "atm_tracers" else 0, implicit=flag)
+        kind, nx1 = 1, 8
     u0, _ = ex.init_state(cfg, kind, SEED, ())
     s0 = None
     if cfg.ntracer:
