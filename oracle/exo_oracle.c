@@ -999,6 +999,37 @@ static void restore_from_perturbation(const or_solver *s, double *wcol, or_colum
   }
 }
 
+/* test shim: the decomposition of column (p, k, j) of the current primitive
+ * state.  wpert (nv, nc1): the column after ChangeToPerturbation; psf
+ * (nc1 + 1): the face pressures of the scan; wrest (nv, nc1): the column
+ * after RestoreFromPerturbation; face_p, face_rho (nc1 + 1, faces is..ie+1):
+ * the restored face states when the reconstructed perturbations are zero. */
+void or_decompose_column(const or_solver *s, int p, int k, int j, double *wpert, double *psf,
+                         double *wrest, double *face_p, double *face_rho) {
+  int nc1 = s->nc1, nv = s->nv;
+  const or_panel *P = &s->pb[p];
+  or_column col;
+  col.psf = psf;
+  col.pres = malloc(sizeof(double) * nc1);
+  col.dens = malloc(sizeof(double) * nc1);
+  double *wl = calloc((size_t)nv * (nc1 + 1), sizeof(double));
+  double *wr = calloc((size_t)nv * (nc1 + 1), sizeof(double));
+  for (int n = 0; n < nv; ++n)
+    for (int i = 0; i < nc1; ++i) wpert[(size_t)n * nc1 + i] = P->w[C4(n, k, j, i)];
+  change_to_perturbation(s, P, wpert, &col);
+  memcpy(wrest, wpert, sizeof(double) * nv * nc1);
+  restore_from_perturbation(s, wrest, &col, wl, wr, s->is, s->ie + 1);
+  size_t st = nc1 + 1;
+  for (int i = s->is; i <= s->ie + 1; ++i) {
+    face_p[i] = wl[IPR * st + i];
+    face_rho[i] = wl[IDN * st + i];
+  }
+  free(col.pres);
+  free(col.dens);
+  free(wl);
+  free(wr);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Hydro::CalculateFluxes with patches/19.calculate_fluxes.patch:17-163 and  */
 /* the CubedSphere L/R save / sync / load (cubed_sphere.cpp:267-687).        */
@@ -1939,6 +1970,13 @@ void or_stage(or_solver *s, int stage, double dt) {
 void or_add_flux_divergence(or_solver *s, double wght) {
   clear_stage_registers(s);
   stage_update(s, 0, wght);
+}
+
+/* the same phase for SSPRK3 stage 1..3: the stage's weighted average of u
+ * and u1, then the tendency with weight wght = beta dt, as or_stage does */
+void or_add_flux_divergence_stage(or_solver *s, int stage, double wght) {
+  if (stage == 1) clear_stage_registers(s);
+  stage_update(s, stage - 1, wght);
 }
 
 void or_implicit_correct(or_solver *s, double dt_stage) { stage_implicit(s, dt_stage); }

@@ -105,6 +105,7 @@ void or_stage(or_solver *s, int stage, double dt); /* one SSPRK3 stage */
 /* single phases of a stage: -div F + sources times wght with stage-1 weights
  * (into u, or into du for the ideal gas), and the implicit correction */
 void or_add_flux_divergence(or_solver *s, double wght);
+void or_add_flux_divergence_stage(or_solver *s, int stage, double wght);
 void or_implicit_correct(or_solver *s, double dt_stage);
 double or_step(or_solver *s, int nsteps, double dt_fixed); /* returns last dt */
 /* the main loop's redo path (main.cpp.in:126-160): nsteps accepted steps,
@@ -125,6 +126,11 @@ void or_riemann_face(const or_config *cfg, int ivx, const double *wl,
                      const double *wr, double *flx, double cth_face,
                      double cos_face, double sin_face, double cos_cell);
 
+/* the hydrostatic decomposition of one column of the current state
+ * (change_to_perturbation.cpp:39-190): perturbations, face pressures, the
+ * restored column and the restored face states of zero perturbations */
+void or_decompose_column(const or_solver *s, int p, int k, int j, double *wpert, double *psf,
+                         double *wrest, double *face_p, double *face_rho);
 /* pins of the implicit-solve algebra and the vector maps */
 void or_flux_jacobian(double gm1, const double *w, double *out25);
 void or_eigensystem(const double *prim, double cs, double gm1, double *R25, double *Ri25,

@@ -39,64 +39,73 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        V = C.c_void_p
-        sig = {
-            "or_weno5": (C.c_double, [C.c_double] * 5),
-            "or_weno3": (C.c_double, [C.c_double] * 3),
-            "or_exp": (C.c_double, [C.c_double]),
-            "or_log": (C.c_double, [C.c_double]),
-            "or_pow": (C.c_double, [C.c_double, C.c_double]),
-            "or_create": (V, [C.POINTER(OrConfig)]),
-            "or_destroy": (None, [V]),
-            "or_set_cons": (None, [V, _DP]),
-            "or_get_cons": (None, [V, _DP]),
-            "or_get_prim": (None, [V, _DP]),
-            "or_get_scalar_prim": (None, [V, _DP]),
-            "or_set_scalars": (None, [V, _DP]),
-            "or_get_scalars": (None, [V, _DP]),
-            "or_get_geom2d": (None, [V, C.c_int, _DP]),
-            "or_get_coord": (None, [V, C.c_int, C.c_int, _DP]),
-            "or_cell_volume": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
-            "or_face_area": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
-            "or_latlon": (None, [V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
-                                 C.POINTER(C.c_double)]),
-            "or_w92_user_out_point": (None, [C.c_double, C.c_double, C.c_int, C.c_double,
-                                             C.c_double, _DP]),
-            "or_user_output": (None, [V, _DP]),
-            "or_fill_ghosts": (None, [V]),
-            "or_cons_to_prim": (None, [V]),
-            "or_calc_fluxes": (None, [V]),
-            "or_get_flux": (None, [V, C.c_int, _DP]),
-            "or_new_dt": (C.c_double, [V]),
-            "or_stage": (None, [V, C.c_int, C.c_double]),
-            "or_step": (C.c_double, [V, C.c_int, C.c_double]),
-            "or_set_threads": (None, [C.c_int]),
-            "or_add_flux_divergence": (None, [V, C.c_double]),
-            "or_implicit_correct": (None, [V, C.c_double]),
-            "or_step_redo": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_double)]),
-            "or_state_valid": (C.c_int, [V]),
-            "or_implicit_column": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
-                                          C.c_double]),
-            "or_flux_jacobian": (None, [C.c_double, _DP, _DP]),
-            "or_eigensystem": (None, [_DP, C.c_double, C.c_double, _DP, _DP, _DP]),
-            "or_roe_average": (None, [_DP, _DP, C.c_double, _DP]),
-            "or_block_thomas": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
-            "or_implicit_column3": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
-                                           C.c_double]),
-            "or_block_thomas3": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
-            "or_vec_maps": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
-                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
-            "or_riemann_face": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP,
-                                       C.c_double, C.c_double, C.c_double, C.c_double]),
-        }
-        for n, (r, a) in sig.items():
-            f = getattr(L, n)
-            f.restype = r
-            f.argtypes = a
-        _lib = L
+        _lib = load(LIB)
     return _lib
+
+
+def load(path):
+    """The oracle library at path with its entry points typed (lib() loads the
+    built one; the mutation tests load copies built with a planted mistake)."""
+    L = C.CDLL(os.path.abspath(path))
+    V = C.c_void_p
+    sig = {
+        "or_weno5": (C.c_double, [C.c_double] * 5),
+        "or_weno3": (C.c_double, [C.c_double] * 3),
+        "or_exp": (C.c_double, [C.c_double]),
+        "or_log": (C.c_double, [C.c_double]),
+        "or_pow": (C.c_double, [C.c_double, C.c_double]),
+        "or_create": (V, [C.POINTER(OrConfig)]),
+        "or_destroy": (None, [V]),
+        "or_set_cons": (None, [V, _DP]),
+        "or_get_cons": (None, [V, _DP]),
+        "or_get_prim": (None, [V, _DP]),
+        "or_get_scalar_prim": (None, [V, _DP]),
+        "or_set_scalars": (None, [V, _DP]),
+        "or_get_scalars": (None, [V, _DP]),
+        "or_get_geom2d": (None, [V, C.c_int, _DP]),
+        "or_get_coord": (None, [V, C.c_int, C.c_int, _DP]),
+        "or_cell_volume": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "or_face_area": (C.c_double, [V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "or_latlon": (None, [V, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)]),
+        "or_w92_user_out_point": (None, [C.c_double, C.c_double, C.c_int, C.c_double,
+                                         C.c_double, _DP]),
+        "or_user_output": (None, [V, _DP]),
+        "or_fill_ghosts": (None, [V]),
+        "or_cons_to_prim": (None, [V]),
+        "or_calc_fluxes": (None, [V]),
+        "or_get_flux": (None, [V, C.c_int, _DP]),
+        "or_new_dt": (C.c_double, [V]),
+        "or_stage": (None, [V, C.c_int, C.c_double]),
+        "or_step": (C.c_double, [V, C.c_int, C.c_double]),
+        "or_set_threads": (None, [C.c_int]),
+        "or_add_flux_divergence": (None, [V, C.c_double]),
+        "or_add_flux_divergence_stage": (None, [V, C.c_int, C.c_double]),
+        "or_implicit_correct": (None, [V, C.c_double]),
+        "or_step_redo": (C.c_int, [V, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "or_state_valid": (C.c_int, [V]),
+        "or_implicit_column": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
+                                      C.c_double]),
+        "or_flux_jacobian": (None, [C.c_double, _DP, _DP]),
+        "or_eigensystem": (None, [_DP, C.c_double, C.c_double, _DP, _DP, _DP]),
+        "or_roe_average": (None, [_DP, _DP, C.c_double, _DP]),
+        "or_block_thomas": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
+        "or_implicit_column3": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP, _DP,
+                                       C.c_double]),
+        "or_block_thomas3": (None, [C.c_int, _DP, _DP, _DP, _DP, _DP]),
+        "or_vec_maps": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "or_decompose_column": (None, [V, C.c_int, C.c_int, C.c_int, _DP, _DP, _DP, _DP,
+                                       _DP]),
+        "or_riemann_face": (None, [C.POINTER(OrConfig), C.c_int, _DP, _DP, _DP,
+                                   C.c_double, C.c_double, C.c_double, C.c_double]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype = r
+        f.argtypes = a
+    return L
 
 
 def or_config(cfg) -> OrConfig:
@@ -106,9 +115,9 @@ def or_config(cfg) -> OrConfig:
 
 
 class Oracle:
-    def __init__(self, cfg):
+    def __init__(self, cfg, L=None):
         self.cfg = or_config(cfg)
-        self.L = lib()
+        self.L = L if L is not None else lib()
         self.h = self.L.or_create(C.byref(self.cfg))
         self.nv = 4 if self.cfg.eos == EOS_SHALLOW_YZ else 5
         self.nc1, self.nc2 = self.cfg.nx1 + 6, self.cfg.nx + 6
@@ -170,8 +179,11 @@ class Oracle:
     def step(self, nsteps=1, dt=0.0):
         return self.L.or_step(self.h, nsteps, dt)
 
-    def add_flux_divergence(self, wght):
-        self.L.or_add_flux_divergence(self.h, wght)
+    def add_flux_divergence(self, wght, stage=1):
+        if stage == 1:
+            self.L.or_add_flux_divergence(self.h, wght)
+        else:
+            self.L.or_add_flux_divergence_stage(self.h, stage, wght)
 
     def implicit_correct(self, dt_stage):
         self.L.or_implicit_correct(self.h, dt_stage)
