@@ -78,6 +78,7 @@ def lib():
         "exo_calc_fluxes": (C.c_int, [H]),
         "exo_download_flux": (C.c_int, [H, C.c_int, _DP]),
         "exo_add_flux_divergence": (C.c_int, [H, C.c_double]),
+        "exo_integrate_hydro": (C.c_int, [H, C.c_int, C.c_double]),
         "exo_fill_ghosts": (C.c_int, [H]),
         "exo_implicit_correct": (C.c_int, [H, C.c_double]),
         "exo_new_dt": (C.c_int, [H, C.POINTER(C.c_double)]),
@@ -89,6 +90,7 @@ def lib():
         "exo_init_state": (C.c_int, [H, C.c_int, C.c_uint64, C.c_void_p, C.c_int, _DP,
                                      C.c_void_p]),
         "exo_riemann_face": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, _DP, _DP, _DP]),
+        "exo_implicit_column": (C.c_int, [H, C.c_int, _DP, _DP, _DP, _DP, C.c_double]),
         "exo_weno5_batch": (C.c_int, [H, _DP, _DP, C.c_int]),
         "exo_math_batch": (C.c_int, [H, C.c_int, _DP, C.c_double, _DP, C.c_int]),
         "exo_init_state_cfg": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint64, C.c_void_p,
@@ -303,6 +305,12 @@ class Solver:
         """-div F + sources times wght (stage-1 weights) into u, or into the
         tendency du for the ideal gas"""
         self._chk(self.L.exo_add_flux_divergence(self.h, wght), "exo_add_flux_divergence")
+
+    def integrate_hydro(self, stage, dt):
+        """IntegrateHydro of SSPRK3 stage 1..3: the stage's weighted average of
+        u, then -div F + sources times beta(stage) dt into u (shallow water) or
+        the tendency du (ideal gas)"""
+        self._chk(self.L.exo_integrate_hydro(self.h, stage, dt), "exo_integrate_hydro")
 
     def fill_ghosts(self):
         self._chk(self.L.exo_fill_ghosts(self.h), "exo_fill_ghosts")

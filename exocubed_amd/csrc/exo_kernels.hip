@@ -1032,54 +1032,99 @@ constexpr int IMPL_MAXNC1 = 512;  // radial cells incl. ghosts the column solve 
 // stage's weighted average of u (with the stage register u1) is formed here,
 // as update_cell forms it, before u += delta: u is read and written once per
 // stage instead of twice.
+typedef double dpair __attribute__((ext_vector_type(2)));
+
+// Column access of the 5x5 solve on the handle's grid: primitives projected
+// with the column's cos/sin (solve_implicit_3d.cpp:117-123), the tendency du,
+// the x1 face areas and volume of each level, and the end of the backward
+// substitution: u (or, with AVG, the stage's weighted average of u and u1)
+// plus the correction.
 template <bool AVG>
-__device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, int lb,
-                                              const StageW &sw, double *geo) {
-  double beta = sw.beta;
-  for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
-    geo[q] = g.x1f[q];
-    if (q < g.nc1) {
-      geo[IMPL_MAXNC1 + 1 + q] = g.x1v[q];
-      geo[2 * (IMPL_MAXNC1 + 1) + q] = g.dx1f[q];
-    }
-  }
-  __syncthreads();
-  int N = g.N;
-  int nbx = (N * N + 63) / 64;
-  int col = (lb % nbx) * 64 + threadIdx.x;
-  int p = lb / nbx;
-  if (col >= N * N) return;
-  int k = g.ks + col / N, j = g.js + col % N;
-  int is = g.is, ie = g.ie;
-  double dt = beta * st.dtv[0];
-  double gamma = g.gamma, grav = g.grav;
-  // column constants: Grid::area1 = ((DX2A DX3A) x1f) x1f SINC and
-  // Grid::volume = ((((dx1f DX2A) DX3A) x1v) x1v) SINC with the same roundings
-  double cth = g.T(T_COSC, p, k, j), sth = g.T(T_SINC, p, k, j);
-  double dx2a = g.T(T_DX2A, p, k, j), dx3a = g.T(T_DX3A, p, k, j);
-  double a23 = dx2a * dx3a;
-  const double *x1f = geo, *x1v = geo + IMPL_MAXNC1 + 1, *dx1f = geo + 2 * (IMPL_MAXNC1 + 1);
-  const ADDR(1) double *__restrict__ W = st.w;
-  const ADDR(1) double *__restrict__ DU = st.du;
-  ADDR(1) double *__restrict__ U = st.u;
-  // coefficient scratch, [block][level][pair][lane]: the 30 doubles of a level
-  // (a^-1 c row-major, then delta) as 15 pairs, each pair 64 lanes x 16 B
-  // contiguous, so a level is 15 coalesced 16-byte stores, and 15 loads back
-  typedef double dpair __attribute__((ext_vector_type(2)));
-  ADDR(1) dpair *__restrict__ CFP =
-      (ADDR(1) dpair *)st.coef + (size_t)lb * g.nx1 * 15 * 64 + threadIdx.x;
-  auto cfp = [&](int i, int q2) -> ADDR(1) dpair * {
-    return CFP + ((size_t)(i - is) * 15 + q2) * 64;
-  };
-  size_t vs = g.vstride, jp = g.jp;
-  size_t c0 = g.c(0, p, k, 0, j);  // column base (variable 0, i = 0)
-  auto wload = [&](int i, double *w) {
+struct ColIOGrid {
+  const ADDR(1) double *__restrict__ W;
+  const ADDR(1) double *__restrict__ DU;
+  ADDR(1) double *__restrict__ U;
+  ADDR(1) double *__restrict__ U1;
+  size_t vs, jp, c0;
+  double cth, sth, a23, dx2a, dx3a;
+  const double *x1f, *x1v, *dx1f;
+  StageW sw;
+  bool first;
+  __device__ __forceinline__ void wraw(int i, double *w) const {
 #pragma unroll
     for (int n = 0; n < 5; ++n) w[n] = W[n * vs + c0 + (size_t)i * jp];
-  };
-  auto wproj = [&](double *w) {
+  }
+  __device__ __forceinline__ void project(double *w) const {
     w[IVY] += w[IVZ] * cth;
     w[IVZ] *= sth;
+  }
+  __device__ __forceinline__ double du(int i, int n) const {
+    return DU[n * vs + c0 + (size_t)i * jp];
+  }
+  // Grid::area1 = ((DX2A DX3A) x1f) x1f SINC and Grid::volume =
+  // ((((dx1f DX2A) DX3A) x1v) x1v) SINC with the same roundings
+  __device__ __forceinline__ void geom(int i, double &aleft, double &aright, double &v) const {
+    aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
+    aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
+    v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
+  }
+  __device__ __forceinline__ void uload(int l, V5 &u, V5 &u1) const {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
+    if (AVG && !first) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) u1.a[n] = U1[n * vs + c0 + (size_t)l * jp];
+    }
+  }
+  // only u += delta is stored: nothing reads the correction itself afterwards
+  __device__ __forceinline__ void finish(int l, const V5 &uo, const V5 &u1o, const V5 &d) const {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      size_t c = n * vs + c0 + (size_t)l * jp;
+      double ub = AVG ? stage_avg(sw, uo.a[n], first ? 0. : u1o.a[n], U1 + c) : uo.a[n];
+      U[c] = ub + d.a[n];
+    }
+  }
+};
+
+// Column access of the single-column shim: projected primitives w[n][i] and
+// the tendency du[n][i] of nc1 levels (x1 ghosts included), the x1 face areas
+// of the interior faces and the interior volumes; the correction itself is
+// written back into du.
+struct ColIOShim {
+  const double *w;
+  double *dun;
+  const double *area, *vol;
+  int nc1, is;
+  __device__ __forceinline__ void wraw(int i, double *wv) const {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) wv[n] = w[n * nc1 + i];
+  }
+  __device__ __forceinline__ void project(double *) const {}
+  __device__ __forceinline__ double du(int i, int n) const { return dun[n * nc1 + i]; }
+  __device__ __forceinline__ void geom(int i, double &aleft, double &aright, double &v) const {
+    aleft = area[i - is];
+    aright = area[i - is + 1];
+    v = vol[i - is];
+  }
+  __device__ __forceinline__ void uload(int, V5 &, V5 &) const {}
+  __device__ __forceinline__ void finish(int l, const V5 &, const V5 &, const V5 &d) const {
+#pragma unroll
+    for (int n = 0; n < 5; ++n) dun[n * nc1 + l] = d.a[n];
+  }
+};
+
+// The 5x5 column solve (full_correction.cpp:32-165, forward_backward.hpp:
+// 15-126) over levels is..ie of one column, with time step dt.  CFP is the
+// lane's base of the coefficient scratch, [level][pair][64 lanes]: the 30
+// doubles of a level (a^-1 c row-major, then delta) as 15 pairs, each pair 64
+// lanes x 16 B contiguous, so a level is 15 coalesced 16-byte stores, and 15
+// loads back.
+template <bool AVG, typename IO>
+__device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double dt, double gamma,
+                                           double grav, ADDR(1) dpair *__restrict__ CFP) {
+  auto cfp = [&](int i, int q2) -> ADDR(1) dpair * {
+    return CFP + ((size_t)(i - is) * 15 + q2) * 64;
   };
   double gm1c = (gamma - 1.) * 1. / 1.;
   double gm1 = 0.5 * (gm1c + gm1c);
@@ -1088,10 +1133,10 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   // |A| at face is-1/2: the reference's row is-1 (full_correction.cpp:272-325)
   // leaves Am = R|L|R^-1 of the Roe state of (w(is-1), w(is))
   double wm[5], w0[5], wp[5], prim[5];
-  wload(is - 1, wm);
-  wproj(wm);
-  wload(is, w0);
-  wproj(w0);
+  io.wraw(is - 1, wm);
+  io.project(wm);
+  io.wraw(is, w0);
+  io.project(w0);
   roe_average(prim, gm1, wm, w0);
   double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
   M5 Am = abs_a(prim, cs, gm1);
@@ -1104,11 +1149,11 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
   // to use (projected, divided by dt) before the level's coefficient stores
   // are issued, so a level never waits for the previous level's stores.
   double wnext[5], rnext[5], enext;
-  wload(is + 1, wnext);
-  wproj(wnext);
+  io.wraw(is + 1, wnext);
+  io.project(wnext);
   enext = erow(wnext);
 #pragma unroll
-  for (int n = 0; n < 5; ++n) rnext[n] = div_rn(DU[n * vs + c0 + (size_t)is * jp], dt, ydt);
+  for (int n = 0; n < 5; ++n) rnext[n] = div_rn(io.du(is, n), dt, ydt);
   for (int i = is; i <= ie; ++i) {
     double rhsv[5], wnx[5], dun[5];
 #pragma unroll
@@ -1118,13 +1163,12 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
     }
     double ep = enext;
     if (i < ie) {  // prefetch the next level
-      wload(i + 2, wnx);
+      io.wraw(i + 2, wnx);
 #pragma unroll
-      for (int n = 0; n < 5; ++n) dun[n] = DU[n * vs + c0 + (size_t)(i + 1) * jp];
+      for (int n = 0; n < 5; ++n) dun[n] = io.du(i + 1, n);
     }
-    double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
-    double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
-    double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
+    double aleft, aright, v;
+    io.geom(i, aleft, aright, v);
     double v2 = 2. * v, yv2 = 1. / v2;
     // b, then b Aprev and b dprev, so b dies early
     M5 b;
@@ -1182,7 +1226,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
     if (i < ie) {  // use the prefetched next level before this level's stores
 #pragma unroll
       for (int n = 0; n < 5; ++n) wnext[n] = wnx[n];
-      wproj(wnext);
+      io.project(wnext);
       enext = erow(wnext);
 #pragma unroll
       for (int n = 0; n < 5; ++n) rnext[n] = div_rn(dun[n], dt, ydt);
@@ -1207,34 +1251,12 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
       w0[n] = wp[n];
     }
   }
-  // backward substitution; delta(ie) is final.  Only u += delta is stored:
-  // nothing reads the correction itself afterwards.
-  ADDR(1) double *U1 = st.u1;
-  bool first = sw.first;
-  auto uload = [&](int l, V5 &u, V5 &u1) {
-#pragma unroll
-    for (int n = 0; n < 5; ++n) u.a[n] = U[n * vs + c0 + (size_t)l * jp];
-    if (AVG && !first) {
-#pragma unroll
-      for (int n = 0; n < 5; ++n) u1.a[n] = U1[n * vs + c0 + (size_t)l * jp];
-    }
-  };
-  // the state delta is added to: u itself, or the stage's weighted average
-  auto ubase = [&](int l, const V5 &uo, const V5 &u1o) -> V5 {
-    if (!AVG) return uo;
-    V5 ub;
-#pragma unroll
-    for (int n = 0; n < 5; ++n)
-      ub.a[n] = stage_avg(sw, uo.a[n], first ? 0. : u1o.a[n], U1 + n * vs + c0 + (size_t)l * jp);
-    return ub;
-  };
+  // backward substitution; delta(ie) is final
   V5 dnext = dprev;
   {
     V5 uo, u1o;
-    uload(ie, uo, u1o);
-    V5 ub = ubase(ie, uo, u1o);
-#pragma unroll
-    for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)ie * jp] = ub.a[n] + dnext.a[n];
+    io.uload(ie, uo, u1o);
+    io.finish(ie, uo, u1o, dnext);
   }
   // levels are loaded ahead within the 63 loads the vector-memory counter
   // tracks: three levels of 20 loads, or two of 25 with the stage register
@@ -1255,7 +1277,7 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
       for (int q = 0; q < 5; ++q) A.a[r][q] = cv[r * 5 + q];
 #pragma unroll
     for (int n = 0; n < 5; ++n) d.a[n] = cv[25 + n];
-    uload(l, u, u1);
+    io.uload(l, u, u1);
   };
 #pragma unroll
   for (int q = 0; q < PD; ++q)
@@ -1271,12 +1293,49 @@ __device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, 
       V5 ad = m5_mulv(A, dnext);
 #pragma unroll
       for (int n = 0; n < 5; ++n) d.a[n] = d.a[n] - ad.a[n];
-      V5 ub = ubase(i, uo, u1o);
-#pragma unroll
-      for (int n = 0; n < 5; ++n) U[n * vs + c0 + (size_t)i * jp] = ub.a[n] + d.a[n];
+      io.finish(i, uo, u1o, d);
       dnext = d;
     }
   }
+}
+
+template <bool AVG>
+__device__ __forceinline__ void implicit_body(const DGrid &g, const DState &st, int lb,
+                                              const StageW &sw, double *geo) {
+  for (int q = threadIdx.x; q <= g.nc1; q += blockDim.x) {
+    geo[q] = g.x1f[q];
+    if (q < g.nc1) {
+      geo[IMPL_MAXNC1 + 1 + q] = g.x1v[q];
+      geo[2 * (IMPL_MAXNC1 + 1) + q] = g.dx1f[q];
+    }
+  }
+  __syncthreads();
+  int N = g.N;
+  int nbx = (N * N + 63) / 64;
+  int col = (lb % nbx) * 64 + threadIdx.x;
+  int p = lb / nbx;
+  if (col >= N * N) return;
+  int k = g.ks + col / N, j = g.js + col % N;
+  ColIOGrid<AVG> io;
+  io.W = st.w;
+  io.DU = st.du;
+  io.U = st.u;
+  io.U1 = st.u1;
+  io.vs = g.vstride;
+  io.jp = g.jp;
+  io.c0 = g.c(0, p, k, 0, j);  // column base (variable 0, i = 0)
+  io.cth = g.T(T_COSC, p, k, j);
+  io.sth = g.T(T_SINC, p, k, j);
+  io.dx2a = g.T(T_DX2A, p, k, j);
+  io.dx3a = g.T(T_DX3A, p, k, j);
+  io.a23 = io.dx2a * io.dx3a;
+  io.x1f = geo;
+  io.x1v = geo + IMPL_MAXNC1 + 1;
+  io.dx1f = geo + 2 * (IMPL_MAXNC1 + 1);
+  io.sw = sw;
+  io.first = sw.first;
+  ADDR(1) dpair *CFP = (ADDR(1) dpair *)st.coef + (size_t)lb * g.nx1 * 15 * 64 + threadIdx.x;
+  col5_solve<AVG>(io, g.is, g.ie, sw.beta * st.dtv[0], g.gamma, g.grav, CFP);
 }
 
 template <bool AVG>
@@ -1287,6 +1346,14 @@ __global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Wo
     implicit_body<AVG>(g, st, lb, sw, geo);
     __syncthreads();  // geo is refilled by the next block of the loop
   }
+}
+
+// One column through the same solve (the single-column shim): lane 0 of one
+// wavefront; coef holds the lane's [level][pair][64 lanes] scratch.
+__global__ void __launch_bounds__(64) k_implicit_column(ColIOShim io, int is, int ie, double dt,
+                                                        double gamma, double grav, double *coef) {
+  if (threadIdx.x != 0) return;
+  col5_solve<false>(io, is, ie, dt, gamma, grav, (ADDR(1) dpair *)glb(coef));
 }
 
 // ImplicitSolver::PartialCorrection on an x1 column (partial_correction.cpp:
@@ -3113,6 +3180,18 @@ int exo_add_flux_divergence(exo_handle *h, double wght) {
   });
 }
 
+int exo_integrate_hydro(exo_handle *h, int stage, double dt) {
+  return guarded(h, [&] {
+    if (stage < 1 || stage > 3) throw std::string("stage must be 1..3");
+    // the stage's weighted average of u and u1, then the update with
+    // wght = beta(stage) dt (the kernel forms beta dt from the dt it reads)
+    double dtv = dt;
+    HIPCHK(hipMemcpyAsync(h->st.dtv, &dtv, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    launch_update(h->L, stage, 0);
+    HIPCHK(hipStreamSynchronize(h->stream));
+  });
+}
+
 int exo_fill_ghosts(exo_handle *h) {
   return guarded(h, [&] {
     launch_fill_ghosts(h->L);
@@ -3494,6 +3573,32 @@ int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *
                        d + 5, d + 10);
     HIPCHK(hipMemcpyAsync(flx, d + 10, h->g.nv * sizeof(double), hipMemcpyDeviceToHost,
                           h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipFree(d));
+  });
+}
+
+int exo_implicit_column(exo_handle *h, int nx1, const double *w_col, double *du_col,
+                        const double *area1, const double *vol, double dt) {
+  return guarded(h, [&] {
+    if (h->g.eos != EOS_IDEAL) throw std::string("the column solve needs the ideal-gas EOS");
+    if (nx1 < 1 || nx1 + 2 * NG > IMPL_MAXNC1) throw std::string("bad column length");
+    if (!w_col || !du_col || !area1 || !vol) throw std::string("null column array");
+    int nc1 = nx1 + 2 * NG;
+    size_t nw = (size_t)5 * nc1, ncoef = (size_t)nx1 * 15 * 64 * 2;
+    double *d;
+    HIPCHK(hipMalloc(&d, (ncoef + 2 * nw + 2 * (size_t)nx1 + 1) * sizeof(double)));
+    double *dw = d + ncoef, *ddu = dw + nw, *da = ddu + nw, *dv = da + nx1 + 1;
+    HIPCHK(hipMemcpyAsync(dw, w_col, nw * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(ddu, du_col, nw * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(da, area1, (nx1 + 1) * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
+    HIPCHK(hipMemcpyAsync(dv, vol, nx1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ColIOShim io{dw, ddu, da, dv, nc1, NG};
+    hipLaunchKernelGGL(k_implicit_column, dim3(1), dim3(64), 0, h->stream, io, NG, NG + nx1 - 1,
+                       dt, h->g.gamma, h->g.grav, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(du_col, ddu, nw * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(d));
   });

@@ -106,8 +106,18 @@ EXO_API int exo_download_flux(exo_handle *h, int dir, double *out);
 /* Hydro::AddFluxDivergence(wght, u) followed by
  * GnomonicEquiangle::AddCoordTermsDivergence (gnomonic_equiangle.cpp:834-879)
  * into u (shallow water) or into the tendency du (ideal gas, zeroed first, as
- * ImplicitHydroTasks::IntegrateHydro, implicit_hydro_tasks.cpp:157-162). */
+ * ImplicitHydroTasks::IntegrateHydro, implicit_hydro_tasks.cpp:157-162).
+ * It applies the first SSPRK3 stage's register weights (u1 <- u, u kept): at
+ * stages 2 and 3 use exo_integrate_hydro, which forms that stage's average. */
 EXO_API int exo_add_flux_divergence(exo_handle *h, double wght);
+/* ImplicitHydroTasks::IntegrateHydro (implicit_hydro_tasks.cpp:128-186) with
+ * AddSourceTerms (:188-214) at SSPRK3 stage 1..3: the stage's weighted average
+ * of u and the register u1 (SSPRK3 Shu-Osher weights), then
+ * AddFluxDivergence + AddCoordTermsDivergence + sources with
+ * wght = beta(stage) dt into du (ideal gas) or u (shallow water).  Followed by
+ * exo_implicit_correct(h, beta(stage) dt) and exo_fill_ghosts it gives the
+ * same bits as exo_stage(h, stage, dt). */
+EXO_API int exo_integrate_hydro(exo_handle *h, int stage, double dt);
 /* CubedSphereUtility::PackData panel-edge remap (cubed_sphere_utility.cpp:
  * 371-617) of the conserved state into the face ghost zones, then
  * cons->prim (EquationOfState::ConservedToPrimitive) and the reflecting x1 BC. */
@@ -242,6 +252,15 @@ EXO_API int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, c
  * handle's grid is used). */
 EXO_API int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const double *wl,
                      const double *wr, double *flx);
+/* ImplicitSolver::FullCorrection on one x1 column (full_correction.cpp:32-165
+ * with forward_backward.hpp:15-126), computed by the device's column solve:
+ * w_col (5, nx1 + 6) primitives of the column with its x1 ghosts, velocities
+ * already projected as SolveImplicit3D projects them (solve_implicit_3d.cpp:
+ * 117-123); du_col (5, nx1 + 6) the tendency, its interior levels replaced by
+ * the correction; area1 the nx1 + 1 interior x1 face areas, vol the nx1
+ * volumes; gamma and grav of this handle's configuration (ideal gas). */
+EXO_API int exo_implicit_column(exo_handle *h, int nx1, const double *w_col, double *du_col,
+                                const double *area1, const double *vol, double dt);
 /* WENO5 value at the face between b and c seen from cell c (device). */
 EXO_API int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
 /* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y), or

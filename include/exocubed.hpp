@@ -116,6 +116,11 @@ class Hydro {
   void AddFluxDivergence(Real wght) {
     check(exo_add_flux_divergence(h_, wght), h_, "Hydro::AddFluxDivergence");
   }
+  // ImplicitHydroTasks::IntegrateHydro + AddSourceTerms of SSPRK3 stage 1..3:
+  // the stage's weighted average of u, then the update with beta(stage) dt
+  void IntegrateHydro(int stage, Real dt) {
+    check(exo_integrate_hydro(h_, stage, dt), h_, "Hydro::IntegrateHydro");
+  }
   // the CFL time step of the current state
   Real NewBlockTimeStep() {
     Real dt = 0;

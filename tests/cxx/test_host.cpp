@@ -205,6 +205,16 @@ DEVICE_TEST(implicit_solver, partial_correction_by_parts) {
   std::vector<double> ua = a.Download(), ub = b.Download();
   EXPECT_TRUE(ua == ub);
   EXPECT_TRUE(ua != u0);
+  // stages 2 and 3 by parts: IntegrateHydro forms the stage's weighted average
+  const double beta[3] = {1.0, 0.25, 2.0 / 3.0};
+  for (int stage = 2; stage <= 3; ++stage) {
+    hydro.CalculateFluxes();
+    hydro.IntegrateHydro(stage, dt);
+    exo::ImplicitSolver(a).SolveImplicit3D(beta[stage - 1] * dt);
+    exo::BoundaryValues(a).ApplyPhysicalBoundaries();
+    exo::Mesh(b).DoTaskListOneStage(stage, dt);
+    EXPECT_TRUE(a.Download() == b.Download());
+  }
 }
 
 DEVICE_TEST(mesh, redo_then_checkpoint_restart) {

@@ -1,7 +1,14 @@
 """Mixed-precision form (FP32 perturbation fluxes, FP64 reference pressure,
 decomposition, implicit solve and state) against the FP64 GPU form, which is
-itself bit-exact against the oracle.  Bar: relative L2 error <= 1e-5 in every
-conserved variable (the bar the benchmark is quoted at)."""
+itself bit-exact against the oracle.
+
+Bar: relative L2 error <= 1e-5 in every conserved variable (the bar the
+benchmark is quoted at).  The noisy 3D atmosphere amplifies any perturbation,
+so there the bar is checked against the input-rounding floor: the change of
+the FP64 result when the initial state is perturbed at FP32 rounding (2^-24).
+Every variable whose floor is below 2.5e-6 must meet 1e-5 outright; only the
+x1 momentum, whose floor is above that, may instead stay within 4x its
+floor (check_atmosphere_bar)."""
 import numpy as np
 import pytest
 
@@ -54,18 +61,49 @@ def input_rounding_floor(ex, cfg, kind, nsteps, nx1, params=()):
     return rel_l2(interior(out[1], nx1), interior(out[0], nx1))
 
 
+FLOOR_ESCAPE = 2.5e-6  # a floor above this leaves too little room under 1e-5
+
+
+def check_atmosphere_bar(e, floor):
+    """per conserved variable (rho, m1, m2, m3, E): <= 1e-5 wherever the
+    input-rounding floor is below 2.5e-6; the x1 momentum alone may instead
+    stay within 4x a floor above that"""
+    for v in range(5):
+        if floor[v] < FLOOR_ESCAPE:
+            assert e[v] <= TOL_L2, (v, e, floor)
+        else:
+            assert v == 1, ("only the x1 momentum may use the floor", v, e, floor)
+            assert e[v] <= 4.0 * floor[v], (v, e, floor)
+
+
 @pytest.mark.parametrize("rsolver,implicit", [(3, 9), (2, 9), (3, 1)])
 def test_mixed_atmosphere_l2(ex, rsolver, implicit):
     """The noisy atmosphere amplifies any perturbation (local lapse rates beyond
-    the adiabatic one); the mixed form must stay within 1e-5 or within a small
-    factor of what FP32 rounding of the input alone does."""
+    the adiabatic one): the bar of check_atmosphere_bar on a 6 x 16 x 16 x 16
+    sphere after 20 steps."""
     cfg = atm_config(ex, nx=16, nx1=16, rsolver=rsolver, implicit=implicit)
     (a, b), dts = _pair(ex, cfg, 1, 20)
     e = rel_l2(interior(b[0], 16), interior(a[0], 16))
     floor = input_rounding_floor(ex, cfg, 1, 20, 16)
     print("rel L2 per variable", e, "input-rounding floor", floor, "dt", dts)
     assert np.all(np.isfinite(b[0]))
-    assert np.all((e <= TOL_L2) | (e <= 4.0 * floor)), (e, floor)
+    check_atmosphere_bar(e, floor)
+
+
+def test_mixed_atmosphere_96x64_20_steps(ex):
+    """The benchmark's own workload (6 x 96 x 96 x 64, LMARS, implicit 5x5,
+    1 % temperature noise) after 20 steps, with the bar of
+    check_atmosphere_bar; the measured figures are printed."""
+    cfg = ex.make_config(nx=96, nx1=64, eos=ex.EOS_IDEAL, rsolver=ex.RS_LMARS, implicit_flag=9,
+                         cfl_x1=1, x1min=6371000.0, x1max=6371000.0 + 30000.0, gamma=1.4,
+                         grav=-9.81, omega=7.292e-5, cfl=0.4)
+    params = (250.0, 1.0e5, 287.0, 0.01, 5.0)
+    (a, b), dts = _pair(ex, cfg, 1, 20, params)
+    e = rel_l2(interior(b[0], 64), interior(a[0], 64))
+    del a, b
+    floor = input_rounding_floor(ex, cfg, 1, 20, 64, params)
+    print("6x96x96x64, 20 steps: rel L2 per variable", e, "input-rounding floor", floor)
+    check_atmosphere_bar(e, floor)
 
 
 @pytest.mark.parametrize("rsolver", [0, 1])

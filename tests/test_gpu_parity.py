@@ -178,3 +178,38 @@ def test_device_matches_golden_fixtures(ex):
         s.close()
         for n in range(u.shape[1]):
             assert math.fsum(u[:, n].ravel()) == ref[name]["sum"][n], (name, n)
+
+
+def test_implicit_column_shim_matches_oracle(ex, orc):
+    """The single-column shim of the 5x5 column solve (exo_implicit_column)
+    against the oracle's FullCorrection column (or_implicit_column), bit for
+    bit: stratified columns of 1, 8 and 64 levels with random density and
+    pressure noise, velocities and tendencies, at three time steps."""
+    import ctypes as C
+
+    rng = np.random.default_rng(11)
+    for nx1 in (1, 8, 64):
+        cfg = atm_config(ex, nx=6, nx1=nx1)
+        g = ex.Solver(cfg)
+        oc = orc.or_config(cfg)
+        nc1 = nx1 + 6
+        dz = 30000.0 / nx1
+        z = (np.arange(nc1) - 3 + 0.5) * dz
+        for trial in range(3):
+            w = np.zeros((5, nc1))
+            w[0] = 1.2 * np.exp(-z / 7300.0) * (1 + 0.01 * rng.standard_normal(nc1))
+            w[4] = 1e5 * np.exp(-z / 7300.0) * (1 + 0.01 * rng.standard_normal(nc1))
+            w[1:4] = rng.normal(scale=5.0, size=(3, nc1))
+            zf = np.arange(nx1 + 1) * dz
+            area = 1.0e10 * (1 + zf / 6.371e6) ** 2
+            vol = np.full(nx1, 1.0e10 * dz) * (1 + rng.uniform(0, 1e-3, nx1))
+            du = rng.normal(scale=1e-2, size=(5, nc1)) * np.abs(w) + 1e-6
+            for dt in (0.5, 20.0, 400.0):
+                dg, do = du.copy(), du.copy()
+                assert g.L.exo_implicit_column(g.h, nx1, np.ascontiguousarray(w), dg, area,
+                                               vol, dt) == 0
+                orc.lib().or_implicit_column(C.byref(oc), nx1, np.ascontiguousarray(w), do,
+                                             area, vol, dt)
+                assert np.array_equal(dg, do), (nx1, trial, dt, np.abs(dg - do).max())
+                assert not np.array_equal(dg, du)
+        g.close()

@@ -138,10 +138,12 @@ def test_three_or_more_tracers(ex, orc, which, nt):
 
 @pytest.mark.parametrize("which", ["sw", "atm", "atm_tracers", "atm_partial", "atm_explicit"])
 def test_stage_by_entry_points(ex, orc, which):
-    """One stage through the boundary's single phases (exo_calc_fluxes,
-    exo_add_flux_divergence, exo_implicit_correct, exo_fill_ghosts), each
-    against the oracle's phase, and equal to exo_stage (whose update leaves the
-    weighted average of u to the column solve or to u += du)."""
+    """A full SSPRK3 step through the boundary's single phases
+    (exo_calc_fluxes, exo_add_flux_divergence at stage 1 and
+    exo_integrate_hydro at stages 2 and 3, exo_implicit_correct,
+    exo_fill_ghosts), each phase against the oracle's phase, and each stage
+    equal to exo_stage (whose update leaves the weighted average of u to the
+    column solve or to u += du)."""
     if which == "sw":
         cfg, kind, nx1 = sw_config(ex, nx=8, rsolver=1), 0, 3
     else:
@@ -160,27 +162,33 @@ def test_stage_by_entry_points(ex, orc, which):
     f = ex.Solver(cfg)
     f.upload(u0, s0)
     dt = g.new_dt()
-    g.calc_fluxes()
-    o.calc_fluxes()
-    g.add_flux_divergence(dt)
-    o.add_flux_divergence(dt)
-    if cfg.eos == ex.EOS_SHALLOW_YZ:   # the update went into u
-        assert np.array_equal(interior(g.download()[0], nx1), interior(o.download()[0], nx1))
-    g.implicit_correct(dt)
-    o.implicit_correct(dt)
-    (ug, sg), (uo, so) = g.download(), o.download()
-    assert np.array_equal(interior(ug, nx1), interior(uo, nx1))
-    if cfg.ntracer:
-        assert np.array_equal(interior(sg, nx1), interior(so, nx1))
-    g.fill_ghosts()
-    o.fill_ghosts()
-    ug, uo = g.download()[0], o.download()[0]
-    edge = (slice(None), slice(None), slice(3, -3), slice(0, 3), slice(3, 3 + nx1))
-    assert np.array_equal(ug[edge], uo[edge])          # a ghost strip
-    wg, wo = g.prim(), o.prim()
-    assert np.array_equal(interior(wg, nx1), interior(wo, nx1))
-    f.stage(1, dt)
-    assert np.array_equal(interior(f.download()[0], nx1), interior(ug, nx1))
+    beta = (1.0, 0.25, 2.0 / 3.0)
+    for stage in (1, 2, 3):
+        wght = beta[stage - 1] * dt
+        g.calc_fluxes()
+        o.calc_fluxes()
+        if stage == 1:
+            g.add_flux_divergence(dt)
+        else:
+            g.integrate_hydro(stage, dt)
+        o.add_flux_divergence(wght, stage)
+        if cfg.eos == ex.EOS_SHALLOW_YZ:   # the update went into u
+            assert np.array_equal(interior(g.download()[0], nx1), interior(o.download()[0], nx1))
+        g.implicit_correct(wght)
+        o.implicit_correct(wght)
+        (ug, sg), (uo, so) = g.download(), o.download()
+        assert np.array_equal(interior(ug, nx1), interior(uo, nx1)), stage
+        if cfg.ntracer:
+            assert np.array_equal(interior(sg, nx1), interior(so, nx1)), stage
+        g.fill_ghosts()
+        o.fill_ghosts()
+        ug, uo = g.download()[0], o.download()[0]
+        edge = (slice(None), slice(None), slice(3, -3), slice(0, 3), slice(3, 3 + nx1))
+        assert np.array_equal(ug[edge], uo[edge]), stage          # a ghost strip
+        wg, wo = g.prim(), o.prim()
+        assert np.array_equal(interior(wg, nx1), interior(wo, nx1)), stage
+        f.stage(stage, dt)
+        assert np.array_equal(interior(f.download()[0], nx1), interior(ug, nx1)), stage
     for h in (g, o, f):
         h.close()
 
