@@ -90,11 +90,16 @@ def step_compulsory_bytes_per_cell(nv, mixed):
 PMC_NAMES = {
     "fp64": {"implicit": "k_implicit<true>", "update": "k_update<false, true>",
              "flux_x1": "k_flux_x1", "flux_x2": "k_flux_h<1>", "flux_x3": "k_flux_h<2>",
+             "edge_states": "k_edge_states", "ghost": "k_ghost",
              "prim": "k_prim", "dt": "k_dt"},
     "mixed": {"implicit": "k_implicit<true>", "update": "k_update<true, true>",
               "flux_x1": "k_flux_x1_mix", "flux_x2": "k_flux_h_mix<1, false>",
-              "flux_x3": "k_flux_h_mix<2, false>", "prim": "k_prim", "dt": "k_dt"},
+              "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
+              "dt": "k_dt"},
 }
+# launches of each kernel per SSPRK3 step of the headline workload
+STEP_LAUNCHES = {"implicit": 3, "update": 3, "flux_x1": 3, "flux_x2": 3, "flux_x3": 3,
+                 "edge_states": 3, "ghost": 3, "prim": 3, "dt": 1}
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic_exo3_96x96x64_lmars_implicit.json")
 
 
@@ -120,23 +125,17 @@ def measured_traffic(kernel, form):
 
 def step_traffic_per_cell(form, cells):
     """Measured HBM bytes per cell and step from the counter passes: the sum
-    over one step's dispatches of the form's kernels (three stages, one CFL
-    reduction; the upload's extra ghost fill and primitive pass excluded)"""
+    over one step's launches (STEP_LAUNCHES) of the form's kernels; None if
+    the profile lacks one of them"""
     t = traffic_profile()
     if not t:
         return None
-    per_step = {"implicit": 3, "update": 3, "flux_x1": 3, "flux_x2": 3, "flux_x3": 3,
-                "prim": 3, "dt": 1}
     tot = 0.0
-    for k, n in per_step.items():
+    for k, n in STEP_LAUNCHES.items():
         b = t["kernels"].get(PMC_NAMES[form][k])
         if b is None:
             return None
         tot += n * b["hbm_bytes"]
-    for k in ("k_ghost", "k_edge_states_mix" if form == "mixed" else "k_edge_states"):
-        b = t["kernels"].get(k)
-        if b:
-            tot += 3 * b["hbm_bytes"]
     return tot / cells
 
 
@@ -474,7 +473,10 @@ def recovery_bench(ex, steps=10):
                         "validity read per step (max_redo 3)"}
 
 
-def serving(ex, precision, nreq, keep=2):
+SERVE_MAX_ACTIVE = 8   # chosen from the --serving-sweep measurements
+
+
+def serving(ex, precision, nreq, keep=2, max_active=SERVE_MAX_ACTIVE):
     """Poisson request stream through the continuous-batching server."""
     rng = np.random.default_rng(SEED + 3)
     arrivals = np.cumsum(rng.exponential(1.0 / 40.0, nreq))
@@ -488,7 +490,7 @@ def serving(ex, precision, nreq, keep=2):
                              x1min=R_EARTH, x1max=R_EARTH + 30000.0, gamma=1.4, grav=-9.81,
                              omega=7.292e-5, cfl=0.4, precision=precision)
         reqs.append((cfg, (T0, 1.0e5, 287.0, 0.01, 5.0, 3.0e-3, 0.5)))
-    srv = ex.Server(max_active=4, chunk_steps=20, graph=True, prep_threads=4)
+    srv = ex.Server(max_active=max_active, chunk_steps=20, graph=True, prep_threads=4)
     ids = []
     t0 = time.perf_counter()
     for q, (cfg, par) in enumerate(reqs):
@@ -503,6 +505,7 @@ def serving(ex, precision, nreq, keep=2):
     lat = np.array([r[2]["latency_s"] for r in res])
     return {
         "requests": nreq, "steps_each": 200, "offered_req_per_s": 40.0,
+        "max_active": max_active,
         "cells_mean": float(np.mean([r[2]["cells"] for r in res])),
         "value": st["cell_updates"] / st["span_s"], "unit": "cell-updates/s",
         "latency_p50_s": float(np.percentile(lat, 50)),
@@ -522,7 +525,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--serving-sweep", action="store_true",
+                    help="only the serving stream, both forms, at max_active 4, 8, 16 and 32")
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="only the 256-instance shallow-water batch, both forms (for profiling)")
@@ -540,8 +545,17 @@ def main():
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        # the same bytes through 8-byte loads and stores: a float64 multiply
+        # on views offset by one element, which torch's elementwise kernel
+        # cannot vectorise (one double per lane per access)
+        n = 128 << 20
+        x = torch.ones(n + 1, dtype=torch.float64, device="cuda")[1:]
+        y = torch.empty(n + 1, dtype=torch.float64, device="cuda")[1:]
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_each_way": a.numel() * 4, "copies": 3}))
+        print(json.dumps({"copy_bytes_each_way": a.numel() * 4, "copies": 3,
+                          "f64_scalar_copy_bytes_each_way": n * 8, "f64_copies": 3}))
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -549,6 +563,18 @@ def main():
         sys.exit(2)
     import exocubed_amd as ex
 
+    if args.serving_sweep:
+        rows = []
+        for ma in (4, 8, 16, 32):
+            for name, prec in (("fp64", ex.PREC_FP64), ("mixed", ex.PREC_MIXED)):
+                r, _ = serving(ex, prec, args.requests, keep=0, max_active=ma)
+                rows.append({"max_active": ma, "form": name, "value": r["value"],
+                             "latency_p50_s": r["latency_p50_s"],
+                             "latency_p99_s": r["latency_p99_s"],
+                             "peak_active": r["peak_active"], "peak_bytes": r["peak_bytes"]})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        print(json.dumps({"serving_sweep": rows, "requests": args.requests}))
+        return
     if args.only_ensemble:
         e64, _ = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)
         emx, _ = ensemble(ex, ex.PREC_MIXED, args.ensemble_steps)

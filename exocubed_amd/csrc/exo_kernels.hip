@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/exocubed.h"
@@ -1154,7 +1155,13 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
   enext = erow(wnext);
 #pragma unroll
   for (int n = 0; n < 5; ++n) rnext[n] = div_rn(io.du(is, n), dt, ydt);
-  for (int i = is; i <= ie; ++i) {
+  // One level of the forward sweep.  The first and the last level are
+  // separate instances (FIRST: no previous row and the reflecting bottom fold;
+  // LAST: no prefetch and the reflecting top fold), so the levels between run
+  // without selects, and c is formed after T there, which keeps fewer 5x5
+  // matrices live at once.
+  auto level = [&](auto first_t, auto last_t, int i) {
+    constexpr bool FIRST = decltype(first_t)::value, LAST = decltype(last_t)::value;
     double rhsv[5], wnx[5], dun[5];
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
@@ -1162,7 +1169,7 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
       rhsv[n] = rnext[n];
     }
     double ep = enext;
-    if (i < ie) {  // prefetch the next level
+    if (!LAST) {  // prefetch the next level
       io.wraw(i + 2, wnx);
 #pragma unroll
       for (int n = 0; n < 5; ++n) dun[n] = io.du(i + 1, n);
@@ -1170,7 +1177,8 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
     double aleft, aright, v;
     io.geom(i, aleft, aright, v);
     double v2 = 2. * v, yv2 = 1. / v2;
-    // b, then b Aprev and b dprev, so b dies early
+    // b, then b Aprev and b dprev, so b dies early (the first level keeps it
+    // for the bottom fold)
     M5 b;
     {
       M5 Jm = jac_e(gm1c, wm, em);
@@ -1182,22 +1190,25 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
     }
     M5 ba;
     V5 bd;
-    if (i > is) {
+    if (!FIRST) {
       ba = m5_mul(b, Aprev);
       bd = m5_mulv(b, dprev);
     }
     roe_average(prim, gm1, w0, wp);
     cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
     M5 Ap = abs_a(prim, cs, gm1);
-    M5 c;
-    {
+    auto cmat = [&]() {
+      M5 c;
       M5 Jp = jac_e(gm1c, wp, ep);
 #pragma unroll
       for (int r = 0; r < 5; ++r)
 #pragma unroll
         for (int q = 0; q < 5; ++q)
           c.a[r][q] = div_rn(-(Ap.a[r][q] - Jp.a[r][q]) * aright, v2, yv2);
-    }
+      return c;
+    };
+    M5 c;
+    if (LAST) c = cmat();  // the top fold needs c in T
     M5 T;
     {
       M5 J0 = jac_e(gm1c, w0, e0);
@@ -1211,19 +1222,20 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
                                 (aright - aleft) * J0.a[r][q],
                             v2, yv2) +
                      dtq - phi;
-          if (i == is) a += b.a[r][q] * (q == IVX ? -1. : 1.);  // reflecting bottom
-          if (i == ie) a += c.a[r][q] * (q == IVX ? -1. : 1.);  // reflecting top
-          T.a[r][q] = i > is ? a - ba.a[r][q] : a;
+          if (FIRST) a += b.a[r][q] * (q == IVX ? -1. : 1.);  // reflecting bottom
+          if (LAST) a += c.a[r][q] * (q == IVX ? -1. : 1.);   // reflecting top
+          T.a[r][q] = FIRST ? a : a - ba.a[r][q];
         }
     }
+    if (!LAST) c = cmat();
     V5 tv;
 #pragma unroll
-    for (int n = 0; n < 5; ++n) tv.a[n] = i > is ? rhsv[n] - bd.a[n] : rhsv[n];
+    for (int n = 0; n < 5; ++n) tv.a[n] = FIRST ? rhsv[n] : rhsv[n] - bd.a[n];
     M5 ainv = m5_inv(T);
     V5 delta = m5_mulv(ainv, tv);
     Aprev = m5_mul(ainv, c);
     dprev = delta;
-    if (i < ie) {  // use the prefetched next level before this level's stores
+    if (!LAST) {  // use the prefetched next level before this level's stores
 #pragma unroll
       for (int n = 0; n < 5; ++n) wnext[n] = wnx[n];
       io.project(wnext);
@@ -1250,6 +1262,15 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
       wm[n] = w0[n];
       w0[n] = wp[n];
     }
+  };
+  using T1 = std::integral_constant<bool, true>;
+  using F0 = std::integral_constant<bool, false>;
+  if (is == ie) {
+    level(T1{}, T1{}, is);
+  } else {
+    level(T1{}, F0{}, is);
+    for (int i = is + 1; i < ie; ++i) level(F0{}, F0{}, i);
+    level(F0{}, T1{}, ie);
   }
   // backward substitution; delta(ie) is final
   V5 dnext = dprev;
@@ -1348,12 +1369,24 @@ __global__ void __launch_bounds__(64) k_implicit(const Inst *__restrict__ I_, Wo
   }
 }
 
-// One column through the same solve (the single-column shim): lane 0 of one
-// wavefront; coef holds the lane's [level][pair][64 lanes] scratch.
-__global__ void __launch_bounds__(64) k_implicit_column(ColIOShim io, int is, int ie, double dt,
-                                                        double gamma, double grav, double *coef) {
-  if (threadIdx.x != 0) return;
-  col5_solve<false>(io, is, ie, dt, gamma, grav, (ADDR(1) dpair *)glb(coef));
+// ncol columns through the same solve (the column shims), one lane per
+// column: column c's arrays follow column c - 1's (w and du nc1 levels of 5
+// variables, nx1 + 1 face areas, nx1 volumes); coef holds the coefficient
+// scratch, [64-column block][level][pair][64 lanes], written by each column's
+// forward sweep before its backward substitution reads it.
+__global__ void __launch_bounds__(64) k_implicit_columns(ColIOShim io, int ncol, int nx1,
+                                                         double dt, double gamma, double grav,
+                                                         double *coef) {
+  int c = blockIdx.x * 64 + threadIdx.x;
+  if (c >= ncol) return;
+  ColIOShim cio = io;
+  cio.w += (size_t)c * 5 * io.nc1;
+  cio.dun += (size_t)c * 5 * io.nc1;
+  cio.area += (size_t)c * (nx1 + 1);
+  cio.vol += (size_t)c * nx1;
+  ADDR(1) dpair *CFP =
+      (ADDR(1) dpair *)glb(coef) + (size_t)blockIdx.x * nx1 * 15 * 64 + threadIdx.x;
+  col5_solve<false>(cio, NG, NG + nx1 - 1, dt, gamma, grav, CFP);
 }
 
 // ImplicitSolver::PartialCorrection on an x1 column (partial_correction.cpp:
@@ -3595,8 +3628,8 @@ int exo_implicit_column(exo_handle *h, int nx1, const double *w_col, double *du_
                           h->stream));
     HIPCHK(hipMemcpyAsync(dv, vol, nx1 * sizeof(double), hipMemcpyHostToDevice, h->stream));
     ColIOShim io{dw, ddu, da, dv, nc1, NG};
-    hipLaunchKernelGGL(k_implicit_column, dim3(1), dim3(64), 0, h->stream, io, NG, NG + nx1 - 1,
-                       dt, h->g.gamma, h->g.grav, d);
+    hipLaunchKernelGGL(k_implicit_columns, dim3(1), dim3(64), 0, h->stream, io, 1, nx1, dt,
+                       h->g.gamma, h->g.grav, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(du_col, ddu, nw * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -14,16 +14,25 @@
  *   pinned by closed forms   : geometry (areas/volumes), vector transforms,
  *                              ghost remap of constant / smooth fields,
  *                              block-Thomas vs dense solve (5x5 and 3x3), the
-                              3x3 partial correction against the decoupled 5x5
-                              one, Riemann consistency,
+ *                              3x3 partial correction against the decoupled 5x5
+ *                              one, Riemann consistency,
  *                              W92 user outputs (known answers, rigid lat-lon
  *                              map, wind speed and direction), the redo path
  *                              (plain stepping without failures; a redone step
  *                              is a step at dt / 2^k from the restored state)
- *   parity unpinned          : Athena-side semantics not in the reference tree
- *                              (AddFluxDivergence, SSPRK3 weight order,
- *                              NewBlockTimeStep, reflecting BC fill) -- restated
- *                              from upstream Athena++ conventions.
+ *   pinned by physics        : (tests/oracle_checks.py, each shown to catch
+ *                              planted mistakes by tests/test_oracle_mutants.py)
+ *                              the vector remap of a solid-body rotation, the
+ *                              L/R exchange, the hydrostatic decomposition,
+ *                              Riemann closed forms (Toro ch. 9-10), Coriolis in
+ *                              Williamson test 2, the CFL closed form, the
+ *                              SSPRK3 composition and finite-volume mass change,
+ *                              column conservation of the implicit correction
+ *   parity unpinned          : rounding order only, Athena-side semantics not in
+ *                              the reference tree (the operation order inside
+ *                              AddFluxDivergence and the SSPRK3 average,
+ *                              Mesh::NewTimeStep's x2 limit, the reflecting BC
+ *                              fill) -- restated from upstream Athena++.
  */
 #ifndef EXO_ORACLE_H_
 #define EXO_ORACLE_H_

@@ -188,10 +188,10 @@ def test_implicit_column_shim_matches_oracle(ex, orc):
     import ctypes as C
 
     rng = np.random.default_rng(11)
+    cfg = atm_config(ex, nx=6, nx1=8)   # gamma and grav of the columns
+    g = ex.Solver(cfg)
+    oc = orc.or_config(cfg)
     for nx1 in (1, 8, 64):
-        cfg = atm_config(ex, nx=6, nx1=nx1)
-        g = ex.Solver(cfg)
-        oc = orc.or_config(cfg)
         nc1 = nx1 + 6
         dz = 30000.0 / nx1
         z = (np.arange(nc1) - 3 + 0.5) * dz
@@ -212,4 +212,4 @@ def test_implicit_column_shim_matches_oracle(ex, orc):
                                              area, vol, dt)
                 assert np.array_equal(dg, do), (nx1, trial, dt, np.abs(dg - do).max())
                 assert not np.array_equal(dg, du)
-        g.close()
+    g.close()
