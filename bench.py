@@ -88,11 +88,11 @@ def step_compulsory_bytes_per_cell(nv, mixed):
 
 # kernel of the live profile -> its name in the rocprofv3 PMC summary, per form
 PMC_NAMES = {
-    "fp64": {"implicit": "k_implicit<true>", "update": "k_update<false, true>",
+    "fp64": {"implicit": "k_implicit<true>", "update": "k_update_na<false, 1>",
              "flux_x1": "k_flux_x1", "flux_x2": "k_flux_h<1>", "flux_x3": "k_flux_h<2>",
              "edge_states": "k_edge_states", "ghost": "k_ghost",
              "prim": "k_prim", "dt": "k_dt"},
-    "mixed": {"implicit": "k_implicit<true>", "update": "k_update<true, true>",
+    "mixed": {"implicit": "k_implicit<true>", "update": "k_update_na<true, 1>",
               "flux_x1": "k_flux_x1_mix", "flux_x2": "k_flux_h_mix<1, false>",
               "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
               "dt": "k_dt"},
@@ -473,7 +473,10 @@ def recovery_bench(ex, steps=10):
                         "validity read per step (max_redo 3)"}
 
 
-SERVE_MAX_ACTIVE = 8   # chosen from the --serving-sweep measurements
+# requests on the device at once: the --serving-sweep measurements (64
+# requests per form) give the same throughput within 3 % at 4, 8, 16 and 32,
+# and the lowest median latency at 4
+SERVE_MAX_ACTIVE = 4
 
 
 def serving(ex, precision, nreq, keep=2, max_active=SERVE_MAX_ACTIVE):

@@ -30,6 +30,13 @@ LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix"
                 "add_du_avg")
 
 
+def _tuning(caps):
+    n = len(LAUNCH_KINDS)
+    out = {k: int(c) for k, c in zip(LAUNCH_KINDS, caps[:n])}
+    out.update({k + "_shape": int(c) for k, c in zip(LAUNCH_KINDS, caps[n:]) if c})
+    return out
+
+
 class Config(C.Structure):
     """Mirror of ``exo_config`` (include/exocubed.h)."""
 
@@ -410,10 +417,13 @@ class Solver:
         return out[:self.nv]
 
     def tune(self, reps=3):
-        caps = np.zeros(len(LAUNCH_KINDS), dtype=np.int32)
+        """tuned persistent widths per launch kind, and "<kind>_shape" block
+        shapes (waves per x1 block, levels per update thread) where not the
+        default"""
+        caps = np.zeros(2 * len(LAUNCH_KINDS), dtype=np.int32)
         self._chk(self.L.exo_tune(self.h, reps, caps.ctypes.data_as(C.c_void_p), len(caps)),
                   "exo_tune")
-        return {k: int(c) for k, c in zip(LAUNCH_KINDS, caps)}
+        return _tuning(caps)
 
 
 class Batch:
@@ -446,13 +456,14 @@ class Batch:
         return out.reshape(len(self.members), 5)
 
     def tune(self, reps=3):
-        caps = np.zeros(len(LAUNCH_KINDS), dtype=np.int32)
+        caps = np.zeros(2 * len(LAUNCH_KINDS), dtype=np.int32)
         self._chk(self.L.exo_batch_tune(self.h, reps, caps.ctypes.data_as(C.c_void_p),
                                         len(caps)), "exo_batch_tune")
-        return {k: int(c) for k, c in zip(LAUNCH_KINDS, caps)}
+        return _tuning(caps)
 
     def set_caps(self, caps):
-        arr = np.array([caps.get(k, 0) for k in LAUNCH_KINDS], dtype=np.int32)
+        arr = np.array([caps.get(k, 0) for k in LAUNCH_KINDS] +
+                       [caps.get(k + "_shape", 0) for k in LAUNCH_KINDS], dtype=np.int32)
         self._chk(self.L.exo_batch_set_caps(self.h, arr.ctypes.data_as(C.c_void_p), len(arr)),
                   "exo_batch_set_caps")
 

@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -435,7 +437,7 @@ __global__ void __launch_bounds__(BLK) k_ghost(const Inst *__restrict__ I_, Work
 // pressure of every column into LDS, then each wave takes a contiguous share
 // of the faces with its own WENO window, so a column's faces are worked on by
 // X1W lanes at once.  Per-face arithmetic is independent of the split.
-constexpr int X1W = 4;
+constexpr int X1W = 4;  // the default; the autotuner may pick 2 or 8 (x1_waves)
 
 __device__ __forceinline__ void psf_scan(const DGrid &g, const DState &st, int p, int k, int j,
                                          double *psf, int lane) {
@@ -463,6 +465,7 @@ __device__ __forceinline__ void psf_scan(const DGrid &g, const DState &st, int p
 #undef PSF
 }
 
+template <int W>
 __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, int lb, double *lds) {
   int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
   int N = g.N;
@@ -487,7 +490,7 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
   }
   // this wave's faces [f0, f1)
   int nf = ie + 2 - is;
-  int f0 = is + (nf * seg) / X1W, f1 = is + (nf * (seg + 1)) / X1W;
+  int f0 = is + (nf * seg) / W, f1 = is + (nf * (seg + 1)) / W;
   // window of 6 cells (f-3 .. f+2) for face f, perturbed IDN / IPR
   double q[6][5];
   auto load = [&](int slot, int i) {
@@ -552,11 +555,12 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64 * X1W) k_flux_x1(const Inst *__restrict__ I_, Work w) {
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_flux_x1(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
-    flux_x1_body(g, st, lb, lds);
+    flux_x1_body<W>(g, st, lb, lds);
     __syncthreads();  // psf is rebuilt by the next block of the loop
   }
 }
@@ -803,7 +807,10 @@ __device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u
 // NOAVG (ideal gas within a stage whose column solve follows): the weighted
 // average of u is left to the column solve's backward substitution, which
 // reads and writes u anyway; here only the tendency is formed.
-template <bool MIX, bool NOAVG = false>
+// NOSCAL: the instance has no passive scalars (the caller checked nt == 0),
+// so the cell's update stores nothing and a caller can form several cells'
+// updates with shared loads.
+template <bool MIX, bool NOAVG = false, bool NOSCAL = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out) {
   double dt = st.dtv[0];
@@ -936,7 +943,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
   }
   // passive scalars: weighted average and flux divergence into s
-  for (int n = 0; n < g.nt; ++n) {
+  for (int n = 0; n < (NOSCAL ? 0 : g.nt); ++n) {
     size_t c = g.c(n, p, k, i, j);
     double s = st.s[c], s1;
     if (sw.first) {
@@ -1001,6 +1008,53 @@ __global__ void __launch_bounds__(BLK) k_update(const Inst *__restrict__ I_, Wor
   WORK_LOOP(w) {
     WORK_INST(w);
     update_body<MIX, NOAVG>(g, st, lb, sw);
+  }
+}
+
+// The ideal gas's update within a stage (the tendency only, as
+// update_body<MIX, true>), LV consecutive levels of one column per thread
+// (UPD_NA_L by default, 1, 2 or 4 as the autotuner picks).  With more levels
+// per thread the levels' tendencies are all formed before any is stored, so
+// loads the levels share are issued once; measured on 6 x 96 x 96 x 64 one
+// level per thread is fastest (more cells in flight per SIMD).
+constexpr int UPD_NA_L = 1;
+
+template <bool MIX, int LV>
+__device__ __forceinline__ void update_na_body(const DGrid &g, const DState &st, int lb,
+                                               StageW sw) {
+  size_t t = (size_t)lb * BLK + threadIdx.x;
+  if (t == 0) *st.bad = 0;  // as update_body resets it
+  int N = g.N, nch = (g.nx1 + LV - 1) / LV;
+  size_t total = (size_t)6 * N * nch * N;
+  if (t >= total) return;
+  int j = g.js + (int)(t % N);
+  size_t r = t / N;
+  int i0 = g.is + (int)(r % nch) * LV;
+  r /= nch;
+  int k = g.ks + (int)(r % N), p = (int)(r / N);
+  double out[LV][5];
+  if (g.nt == 0) {
+#pragma unroll
+    for (int q = 0; q < LV; ++q)
+      if (i0 + q <= g.ie) update_cell<MIX, true, true>(g, st, p, k, i0 + q, j, sw, out[q]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < LV; ++q)
+      if (i0 + q <= g.ie) update_cell<MIX, true>(g, st, p, k, i0 + q, j, sw, out[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < LV; ++q)
+    if (i0 + q <= g.ie)
+#pragma unroll
+      for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i0 + q, j)] = out[q][n];
+}
+
+template <bool MIX, int LV>
+__global__ void __launch_bounds__(BLK) k_update_na(const Inst *__restrict__ I_, Work w,
+                                                   StageW sw) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    update_na_body<MIX, LV>(g, st, lb, sw);
   }
 }
 
@@ -1916,6 +1970,7 @@ __device__ __forceinline__ double ln_expm1_ratio(double l) {  // ln((1 - e^-l) /
 // rho_sf(b) = rho_sf(a) e^(-lambda / gamma) and
 // rho_sv = rho_sf(a) ((1 - e^-lambda) / lambda)^(1/gamma).  Outside the
 // series' range a cell falls back to the exact log / pow.
+template <int W>
 __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &st, int lb,
                                                  double *lds) {
 #pragma clang fp contract(fast)
@@ -1933,7 +1988,7 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
   __syncthreads();
   if (!act) return;
   int nf = ie + 2 - is;
-  int f0 = is + (nf * seg) / X1W, f1 = is + (nf * (seg + 1)) / X1W;
+  int f0 = is + (nf * seg) / W, f1 = is + (nf * (seg + 1)) / W;
   double ent0 = g.gamma;
   double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
   double ig = 1. / ent0, e1 = dexp(-ent1 / ent0);
@@ -2028,11 +2083,12 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
 #undef PSF
 }
 
-__global__ void __launch_bounds__(64 * X1W) k_flux_x1_mix(const Inst *__restrict__ I_, Work w) {
+template <int W>
+__global__ void __launch_bounds__(64 * W) k_flux_x1_mix(const Inst *__restrict__ I_, Work w) {
   extern __shared__ double lds[];
   WORK_LOOP(w) {
     WORK_INST(w);
-    flux_x1_mix_body(g, st, lb, lds);
+    flux_x1_mix_body<W>(g, st, lb, lds);
     __syncthreads();  // psf is rebuilt by the next block of the loop
   }
 }
@@ -2403,6 +2459,11 @@ struct Launcher {
   int *d_bstart = nullptr;  // [K_NKIND][ninst + 1]
   int nblk[K_NKIND] = {};
   int cap[K_NKIND] = {};    // workgroups of a persistent launch (0: one per block)
+  int shape[K_NKIND] = {};  // tuned block shape of a kind (0: the default); the x1
+                            // kinds: waves per 64-column block (2, 4 or 8); the
+                            // stage updates: levels per thread (1, 2 or 4)
+  std::vector<Inst> h_inst;     // host copies of the instances and their forms,
+  std::vector<char> h_mixed;    // for block tables rebuilt when a shape changes
   size_t lds_x1 = 0;        // dynamic LDS of the x1 kernels (largest instance)
   // live per-kernel timing with HIP events on the stream
   bool prof = false;
@@ -2481,7 +2542,7 @@ hipEvent_t next_event(Launcher &L) {
 }
 
 // blocks of one instance in a launch of the given kind (0: not applicable)
-int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
+int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, int shape = 0) {
   bool ideal = g.eos == EOS_IDEAL;
   size_t cells = (size_t)6 * g.N * g.N * g.nx1;
   size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
@@ -2506,8 +2567,12 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
-    case K_UPD_NA: return ideal && !mixed ? (int)grid1(cells) : 0;
-    case K_UPD_MIX_NA: return ideal && mixed ? (int)grid1(cells) : 0;
+    case K_UPD_NA:
+    case K_UPD_MIX_NA: {
+      int lv = shape ? shape : UPD_NA_L;
+      bool on = ideal && (kind == K_UPD_MIX_NA) == mixed;
+      return on ? (int)grid1(cells / g.nx1 * ((g.nx1 + lv - 1) / lv)) : 0;
+    }
     case K_IMPL_AVG: return full ? (int)grid1(cols, 64) * 6 : 0;
     case K_UPD_RS: return !ideal ? (int)grid1(cells) : 0;
     case K_IMPL3_AVG:
@@ -2545,11 +2610,14 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     inst[i].ned = (int)h->setup.edges.size();
     L.lds_x1 = std::max(L.lds_x1, (size_t)(h->g.nc1 + 1) * 64 * sizeof(double));
   }
+  L.h_inst = inst;
+  L.h_mixed.assign(n, 0);
+  for (int i = 0; i < n; ++i) L.h_mixed[i] = hs[i]->mixed;
   for (int kind = 0; kind < K_NKIND; ++kind) {
     long long acc = 0;
     for (int i = 0; i < n; ++i) {
       tab[(size_t)kind * (n + 1) + i] = (int)acc;
-      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned);
+      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned, L.shape[kind]);
     }
     if (acc > 0x7fffffffLL) throw std::string("work list too large for one launch");
     tab[(size_t)kind * (n + 1) + n] = (int)acc;
@@ -2579,6 +2647,30 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
   HIPCHK(hipMemcpyAsync(L.d_inst, inst.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, L.stream));
   HIPCHK(hipMemcpyAsync(L.d_bstart, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice,
                         L.stream));
+  HIPCHK(hipStreamSynchronize(L.stream));
+}
+
+// a kind's block shape; the block table of a kind whose block count depends
+// on it (the stage updates) is rebuilt on the device
+void set_shape(Launcher &L, int kind, int shape) {
+  bool x1 = kind == K_X1 || kind == K_X1_MIX, upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
+  bool ok = shape == 0 || (x1 && (shape == 2 || shape == 4 || shape == 8)) ||
+            (upd && (shape == 1 || shape == 2 || shape == 4));
+  if (!ok) throw std::string("no block shape ") + std::to_string(shape) + " for launch kind " +
+               std::to_string(kind);
+  L.shape[kind] = shape;
+  if (kind != K_UPD_NA && kind != K_UPD_MIX_NA) return;
+  int n = L.ninst;
+  std::vector<int> row(n + 1);
+  long long acc = 0;
+  for (int i = 0; i < n; ++i) {
+    row[i] = (int)acc;
+    acc += kind_blocks(kind, L.h_inst[i].g, L.h_mixed[i], L.h_inst[i].ngd, L.h_inst[i].ned, shape);
+  }
+  row[n] = (int)acc;
+  L.nblk[kind] = (int)acc;
+  HIPCHK(hipMemcpyAsync(L.d_bstart + (size_t)kind * (n + 1), row.data(), sizeof(int) * (n + 1),
+                        hipMemcpyHostToDevice, L.stream));
   HIPCHK(hipStreamSynchronize(L.stream));
 }
 
@@ -2758,9 +2850,27 @@ void launch_fill_ghosts(Launcher &L) {
   HIPCHK(hipGetLastError());
 }
 
+// the x1 sweep of one kind with its tuned waves per block
+void launch_x1(Launcher &L, int kind, int kid) {
+  bool mix = kind == K_X1_MIX;
+  switch (L.shape[kind] ? L.shape[kind] : X1W) {
+    case 2:
+      if (mix) launch(L, kind, kid, k_flux_x1_mix<2>, 128, L.lds_x1);
+      else launch(L, kind, kid, k_flux_x1<2>, 128, L.lds_x1);
+      break;
+    case 8:
+      if (mix) launch(L, kind, kid, k_flux_x1_mix<8>, 512, L.lds_x1);
+      else launch(L, kind, kid, k_flux_x1<8>, 512, L.lds_x1);
+      break;
+    default:
+      if (mix) launch(L, kind, kid, k_flux_x1_mix<4>, 256, L.lds_x1);
+      else launch(L, kind, kid, k_flux_x1<4>, 256, L.lds_x1);
+  }
+}
+
 void launch_fluxes(Launcher &L) {
-  launch(L, K_X1, KID_FLUX_X1, k_flux_x1, 64 * X1W, L.lds_x1);
-  launch(L, K_X1_MIX, KID_FLUX_X1, k_flux_x1_mix, 64 * X1W, L.lds_x1);
+  launch_x1(L, K_X1, KID_FLUX_X1);
+  launch_x1(L, K_X1_MIX, KID_FLUX_X1);
   launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
   launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
   launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
@@ -2789,6 +2899,24 @@ void launch_update(Launcher &L, int stage, int prim) {
   HIPCHK(hipGetLastError());
 }
 
+// the ideal gas's stage update (tendency only) with its tuned levels per thread
+void launch_update_na(Launcher &L, int kind, int kid, StageW sw) {
+  bool mix = kind == K_UPD_MIX_NA;
+  switch (L.shape[kind] ? L.shape[kind] : UPD_NA_L) {
+    case 2:
+      if (mix) launch(L, kind, kid, k_update_na<true, 2>, BLK, 0, sw);
+      else launch(L, kind, kid, k_update_na<false, 2>, BLK, 0, sw);
+      break;
+    case 4:
+      if (mix) launch(L, kind, kid, k_update_na<true, 4>, BLK, 0, sw);
+      else launch(L, kind, kid, k_update_na<false, 4>, BLK, 0, sw);
+      break;
+    default:
+      if (mix) launch(L, kind, kid, k_update_na<true, 1>, BLK, 0, sw);
+      else launch(L, kind, kid, k_update_na<false, 1>, BLK, 0, sw);
+  }
+}
+
 // the stage's update and column correction; the instances with the full
 // column solve leave the weighted average of u to it
 void launch_update_implicit(Launcher &L, int stage) {
@@ -2796,8 +2924,8 @@ void launch_update_implicit(Launcher &L, int stage) {
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
             1, DFLOOR, SFLOOR};
   launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
-  launch(L, K_UPD_NA, KID_UPDATE, k_update<false, true>, BLK, 0, sw);
-  launch(L, K_UPD_MIX_NA, KID_UPDATE, k_update<true, true>, BLK, 0, sw);
+  launch_update_na(L, K_UPD_NA, KID_UPDATE, sw);
+  launch_update_na(L, K_UPD_MIX_NA, KID_UPDATE, sw);
   launch(L, K_IMPL_AVG, KID_IMPLICIT, k_implicit<true>, 64, 0, sw);
   launch(L, K_IMPL3_AVG, KID_IMPLICIT, k_implicit3<true>, 64, 0, sw);
   launch(L, K_ADD_DU_AVG, KID_IMPLICIT, k_add_du<true>, BLK, 0, sw);
@@ -2863,45 +2991,64 @@ void tune_launcher(Launcher &L, int reps) {
   HIPCHK(hipEventCreate(&e1));
   bool prof = L.prof;
   L.prof = false;
-  const int kinds[] = {K_PRIM,     K_GHOST_U,  K_X1,          K_X1_MIX, K_EDGE,
-                       K_EDGE_MIX, K_FACE,     K_FACE_MIX,    K_FACE_MIX_SW, K_DT};
+  // the stage updates rewrite the scalars in place: their shape is tuned only
+  // when no instance carries scalars (then repeating one changes nothing)
+  bool scalars = false;
+  for (const Inst &in : L.h_inst) scalars = scalars || in.g.nt > 0;
+  // the updates read the face fluxes of the current state
+  launch_fluxes(L);
+  StageW sw{1.0, 0.0, 1.0, 1.0, 1, 0, DFLOOR, SFLOOR};
+  const int kinds[] = {K_PRIM,     K_GHOST_U, K_X1,          K_X1_MIX, K_EDGE,   K_EDGE_MIX,
+                       K_FACE,     K_FACE_MIX, K_FACE_MIX_SW, K_DT,     K_UPD_NA, K_UPD_MIX_NA};
   for (int kind : kinds) {
-    if (L.nblk[kind] == 0) continue;
-    std::vector<int> cands = {0};
-    for (int m : {1, 2, 4, 8, 16})
-      if ((long long)m * ncu < L.nblk[kind]) cands.push_back(m * ncu);
+    bool upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
+    if (upd && scalars) continue;
+    std::vector<int> shapes = {0};  // the default first
+    if (kind == K_X1 || kind == K_X1_MIX) shapes = {0, 2, 8};
+    if (upd) shapes = {0, 2, 4};
     float best = 1e30f;
-    int best_cap = 0;
-    for (int c : cands) {
-      L.cap[kind] = c;
-      for (int r = -1; r < reps; ++r) {  // r = -1 warms up
-        if (r == 0) HIPCHK(hipEventRecord(e0, L.stream));
-        switch (kind) {
-          case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
-          case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
-          case K_X1: launch(L, K_X1, -1, k_flux_x1, 64 * X1W, L.lds_x1); break;
-          case K_X1_MIX: launch(L, K_X1_MIX, -1, k_flux_x1_mix, 64 * X1W, L.lds_x1); break;
-          case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
-          case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
-          case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
-          case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1, false>, BLK, 0); break;
-          case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
-          case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
+    int best_cap = 0, best_shape = 0;
+    for (int sh : shapes) {
+      set_shape(L, kind, sh);
+      if (L.nblk[kind] == 0) break;
+      std::vector<int> cands = {0};
+      for (int m : {1, 2, 4, 8, 16})
+        if ((long long)m * ncu < L.nblk[kind]) cands.push_back(m * ncu);
+      for (int c : cands) {
+        L.cap[kind] = c;
+        for (int r = -1; r < reps; ++r) {  // r = -1 warms up
+          if (r == 0) HIPCHK(hipEventRecord(e0, L.stream));
+          switch (kind) {
+            case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
+            case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
+            case K_X1: launch_x1(L, K_X1, -1); break;
+            case K_X1_MIX: launch_x1(L, K_X1_MIX, -1); break;
+            case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
+            case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
+            case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
+            case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1, false>, BLK, 0); break;
+            case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
+            case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
+            case K_UPD_NA: case K_UPD_MIX_NA: launch_update_na(L, kind, -1, sw); break;
+          }
+        }
+        HIPCHK(hipEventRecord(e1, L.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best * 0.97f) {  // prefer the default launch unless clearly slower
+          best = ms;
+          best_cap = c;
+          best_shape = sh;
         }
       }
-      HIPCHK(hipEventRecord(e1, L.stream));
-      HIPCHK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best * 0.97f) {  // prefer the plain launch unless clearly slower
-        best = ms;
-        best_cap = c;
-      }
     }
+    set_shape(L, kind, best_shape);
     L.cap[kind] = best_cap;
   }
-  // kinds sharing a shape with a tuned one (update-shaped kinds keep the plain
-  // launch: they stream and cannot be repeated without changing the state)
+  // kinds sharing a shape with a tuned one (the other update-shaped kinds keep
+  // the plain launch: they stream and cannot be repeated without changing the
+  // state)
   L.cap[K_GHOST_S] = L.cap[K_GHOST_U];
   L.prof = prof;
   HIPCHK(hipEventDestroy(e0));
@@ -3807,15 +3954,70 @@ int exo_conserved_sums(exo_handle *h, double *out5) {
   });
 }
 
+namespace {
+static_assert(K_NKIND == EXO_NKINDS, "EXO_NKINDS must count the launch kinds");
+
+// tuned widths and shapes per configuration of the instances of a launcher
+std::mutex g_tune_mu;
+std::map<std::string, std::vector<int>> g_tune_cache;
+
+std::string tune_key(const std::vector<exo_handle *> &hs) {
+  std::string k;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  k += std::to_string(dev);
+  for (const exo_handle *h : hs) {
+    const exo_config &c = h->setup.cfg;
+    k += "|" + std::to_string(c.nx) + "," + std::to_string(c.nx1) + "," + std::to_string(c.eos) +
+         "," + std::to_string(c.rsolver) + "," + std::to_string(c.implicit_flag) + "," +
+         std::to_string(c.ntracer) + "," + std::to_string(c.coriolis) + "," +
+         std::to_string(c.precision);
+  }
+  return k;
+}
+
+// tune a launcher, or apply the result of an earlier tuning of the same
+// configurations
+void tune_cached(Launcher &L, const std::vector<exo_handle *> &hs, int reps) {
+  std::string key = tune_key(hs);
+  std::vector<int> got;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) got = it->second;
+  }
+  if (got.empty()) {
+    tune_launcher(L, reps);
+    got.assign(2 * K_NKIND, 0);
+    for (int q = 0; q < K_NKIND; ++q) {
+      got[q] = L.cap[q];
+      got[K_NKIND + q] = L.shape[q];
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[key] = got;
+  } else {
+    for (int q = 0; q < K_NKIND; ++q) {
+      set_shape(L, q, got[K_NKIND + q]);
+      L.cap[q] = got[q];
+    }
+  }
+}
+
+void report_tuning(const Launcher &L, int *caps, int ncaps) {
+  for (int q = 0; caps && q < ncaps && q < 2 * K_NKIND; ++q)
+    caps[q] = q < K_NKIND ? L.cap[q] : L.shape[q - K_NKIND];
+}
+}  // namespace
+
 int exo_tune(exo_handle *h, int reps, int *caps, int ncaps) {
   return guarded(h, [&] {
     if (!h->have_state) throw std::string("no state uploaded");
-    tune_launcher(h->L, reps > 0 ? reps : 3);
+    tune_cached(h->L, std::vector<exo_handle *>{h}, reps > 0 ? reps : 3);
     if (h->exec) (void)hipGraphExecDestroy(h->exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     h->exec = nullptr;
     h->graph = nullptr;
-    for (int q = 0; caps && q < ncaps && q < K_NKIND; ++q) caps[q] = h->L.cap[q];
+    report_tuning(h->L, caps, ncaps);
   });
 }
 
@@ -3922,8 +4124,8 @@ int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps) {
   return guarded(b, [&] {
     batch_ready(b);
     batch_free_graph(b);
-    tune_launcher(b->L, reps > 0 ? reps : 3);
-    for (int q = 0; caps && q < ncaps && q < K_NKIND; ++q) caps[q] = b->L.cap[q];
+    tune_cached(b->L, b->members, reps > 0 ? reps : 3);
+    report_tuning(b->L, caps, ncaps);
   });
 }
 
@@ -3931,7 +4133,11 @@ int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps) {
   return guarded(b, [&] {
     if (!caps) throw std::string("null caps");
     batch_free_graph(b);
-    for (int q = 0; q < ncaps && q < K_NKIND; ++q) b->L.cap[q] = caps[q] > 0 ? caps[q] : 0;
+    for (int q = 0; q < ncaps && q < 2 * K_NKIND; ++q) {
+      int v = caps[q] > 0 ? caps[q] : 0;
+      if (q < K_NKIND) b->L.cap[q] = v;
+      else set_shape(b->L, q - K_NKIND, v);
+    }
   });
 }
 

@@ -153,11 +153,17 @@ EXO_API int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
  * rounding of each total.  The reference has no such diagnostic; it checks the
  * conservation of the finite-volume update (mass, and energy without sources). */
 EXO_API int exo_conserved_sums(exo_handle *h, double *out5);
-/* Autotune the persistent launch width of the handle's kernels that only read
- * the conserved state (repeated launches change nothing): each candidate
- * width is timed `reps` times and the fastest kept.  caps (may be NULL)
- * receives up to ncaps tuned widths, one per launch kind (0 = one workgroup
- * per block). */
+/* Autotune the launches of the handle's kernels that only read the conserved
+ * state or only write the tendency (repeated launches change nothing): the
+ * persistent launch width, and the block shape where a kernel has several
+ * (waves per 64-column block of the x1 sweep: 2, 4 or 8; levels per thread
+ * of the stage update: 1, 2 or 4, tuned when no passive scalars ride along).
+ * Each candidate is timed `reps` times and the fastest kept; a configuration
+ * tuned before in this process reuses its result without timing.  caps (may
+ * be NULL) receives up to ncaps values: the width of each launch kind (0 = one
+ * workgroup per block), then, from index EXO_NKINDS on, its shape (0 = the
+ * default). */
+#define EXO_NKINDS 27
 EXO_API int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
 
 /* Device bytes exo_create allocates for this configuration (host only; 0 for
@@ -187,7 +193,9 @@ EXO_API const char *exo_batch_last_error(const exo_batch *b);
 EXO_API int exo_batch_step(exo_batch *b, int nsteps, double dt_fixed);
 /* exo_conserved_sums of every member: out[member][5] */
 EXO_API int exo_batch_sums(exo_batch *b, double *out);
-/* exo_tune over the batch's launches; exo_batch_set_caps sets widths directly */
+/* exo_tune over the batch's launches (reused for a batch of the same member
+ * configurations); exo_batch_set_caps sets widths, and from index EXO_NKINDS
+ * on shapes, directly */
 EXO_API int exo_batch_tune(exo_batch *b, int reps, int *caps, int ncaps);
 EXO_API int exo_batch_set_caps(exo_batch *b, const int *caps, int ncaps);
 /* live kernel timing of the batch's launches (kernel ids as exo_profile_end;

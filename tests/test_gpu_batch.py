@@ -79,11 +79,16 @@ def test_batch_persistent_and_tuned(ex):
     ref = alone(ex, specs, 4)
     members = [make(ex, sp, SEED + q) for q, sp in enumerate(specs)]
     b = ex.Batch(members, graph=True)
-    # a very narrow persistent launch: every workgroup loops over many blocks
-    b.set_caps({k: 7 for k in ex.LAUNCH_KINDS})
+    # a very narrow persistent launch: every workgroup loops over many blocks;
+    # the x1 sweeps and the stage updates in their non-default block shapes
+    caps = {k: 7 for k in ex.LAUNCH_KINDS}
+    caps.update({"x1_shape": 8, "x1_mix_shape": 2, "update_na_shape": 4,
+                 "update_mix_na_shape": 1})
+    b.set_caps(caps)
     b.step(2)
     caps = b.tune(reps=2)
-    assert set(caps) == set(ex.LAUNCH_KINDS)
+    assert set(ex.LAUNCH_KINDS) <= set(caps)
+    assert all(caps[k] in (2, 4, 8) for k in ("x1_shape", "x1_mix_shape") if k in caps)
     b.step(2)
     got = [m.download() for m in members]
     b.close()
