@@ -37,6 +37,11 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
 
+def log(msg):
+    """progress on stderr (the JSON line alone goes to stdout)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def workload_config(ex, precision, implicit_flag=9):
     return ex.make_config(nx=96, nx1=64, eos=ex.EOS_IDEAL, rsolver=ex.RS_LMARS,
                           implicit_flag=implicit_flag, ntracer=0, coriolis=ex.COR_NONE, cfl_x1=1,
@@ -584,7 +589,9 @@ def main():
         print(json.dumps({"ensemble": {"fp64": e64, "mixed": emx}}))
         return
     iflag = args.implicit_flag
+    log(f"headline FP64, {args.steps} steps")
     r64 = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, implicit_flag=iflag)
+    log("headline mixed")
     rmx = run_gpu(ex, ex.PREC_MIXED, args.steps, args.warmup, implicit_flag=iflag)
     # the same seeded run in both forms: accuracy of the mixed form vs FP64
     l2 = rel_l2(rmx["u"], r64["u"])
@@ -592,6 +599,7 @@ def main():
     # what FP32 rounding of the input alone does to the FP64 result: the
     # noisy atmosphere amplifies any perturbation, so this is the floor any
     # FP32 computation sits on
+    log("input-rounding floor")
     rfl = run_gpu(ex, ex.PREC_FP64, args.steps, args.warmup, perturb=2.0 ** -24,
                   implicit_flag=iflag)
     floor = rel_l2(rfl["u"], r64["u"])
@@ -639,25 +647,36 @@ def main():
     for k in kernels:
         kernels[k]["share"] = prof[k][0] / sum(x[0] for x in prof.values())
     kernels_mx = kernel_rooflines(rmx["prof"], rmx["cells"], rmx["nv"], True)
+    if not args.no_cpu:
+        log("CPU baseline")
     cpu = None if args.no_cpu else cpu_baseline(workload_config, ex)
     ens = srv = None
     if not args.only_headline:
+        log("ensemble FP64")
         e64, s64 = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)
+        log("ensemble mixed")
         emx, smx = ensemble(ex, ex.PREC_MIXED, args.ensemble_steps)
+        log("ensemble accuracy")
         l2e = max(float(rel_l2(a, b)[[0, 2, 3]].max()) for a, b in zip(smx, s64))
         ens = {"fp64": e64, "mixed": emx, "mixed_over_fp64": emx["value"] / e64["value"],
                "mixed_rel_l2_vs_fp64_max": l2e,
                "accuracy_200_steps": ensemble_accuracy(ex),
                "workload": "256 shallow-water instances, N in {24..192}, 1 layer, WENO5+Roe, "
                            "Coriolis, SSPRK3 CFL 0.4"}
+        log("serving FP64")
         v64, r64s = serving(ex, ex.PREC_FP64, args.requests)
+        log("serving mixed")
         vmx, rmxs = serving(ex, ex.PREC_MIXED, args.requests)
         l2s = max(float(rel_l2(a, b).max()) for a, b in zip(rmxs, r64s))
         srv = {"fp64": v64, "mixed": vmx, "mixed_over_fp64": vmx["value"] / v64["value"],
                "mixed_rel_l2_vs_fp64_max": l2s,
                "workload": "Poisson 40 req/s, 3D ideal gas + 2 tracers, N 32..128, 32..96 "
                            "levels, WENO5+LMARS, implicit 5x5, 200 steps each"}
+    if args.large_steps > 0 and not args.only_headline:
+        log("large run")
     big = large(ex, args.large_steps) if args.large_steps > 0 and not args.only_headline else None
+    if not args.only_headline:
+        log("output, recovery, partial correction")
     outp = None if args.only_headline else output_bench(ex)
     recov = None if args.only_headline else recovery_bench(ex)
     part = None if args.only_headline else partial_correction(ex, min(args.steps, 50),
