@@ -98,6 +98,9 @@ def lib():
                                      C.c_void_p]),
         "exo_riemann_face": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, _DP, _DP, _DP]),
         "exo_implicit_column": (C.c_int, [H, C.c_int, _DP, _DP, _DP, _DP, C.c_double]),
+        "exo_implicit_columns_noalloc": (C.c_int, [H, C.c_int, C.c_int, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_double]),
         "exo_weno5_batch": (C.c_int, [H, _DP, _DP, C.c_int]),
         "exo_math_batch": (C.c_int, [H, C.c_int, _DP, C.c_double, _DP, C.c_int]),
         "exo_init_state_cfg": (C.c_int, [C.POINTER(Config), C.c_int, C.c_uint64, C.c_void_p,

@@ -3169,6 +3169,32 @@ int guarded(T *h, F &&fn) {
   }
 }
 
+// Coefficient scratch of exo_implicit_columns_noalloc: a fixed array in
+// device memory (each GPU the library runs on holds its own copy until the
+// process ends), so that entry point allocates nothing.  Every call on one GPU
+// shares it: calls that overlap on one GPU race.  Nothing in it needs clearing:
+// each column's forward sweep writes every level it holds before the backward
+// substitution reads it.
+static __device__ __align__(256) double scratch[EXO_NOALLOC_SCRATCH_DOUBLES];
+
+// the device address of this GPU's copy of scratch (the array's name in host
+// code is not a device address); looked up once per device, with that device
+// current
+double *scratch_ptr() {
+  static std::mutex mu;
+  static double *ptr[64] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) throw std::string("device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptr[dev] = (double *)p;
+  }
+  return ptr[dev];
+}
+
 // Output sets: fields per cell, the device buffer they go to, the launch.
 size_t output_floats(const Grid &g, int set, int ghost) {
   int G = ghost ? NG : 0;
@@ -3781,6 +3807,27 @@ int exo_implicit_column(exo_handle *h, int nx1, const double *w_col, double *du_
     HIPCHK(hipMemcpyAsync(du_col, ddu, nw * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipFree(d));
+  });
+}
+
+int exo_implicit_columns_noalloc(exo_handle *h, int ncol, int nx1, const double *w_cols,
+                                 double *du_cols, const double *area1, const double *vol,
+                                 double dt) {
+  return guarded(h, [&] {
+    if (h->g.eos != EOS_IDEAL) throw std::string("the column solve needs the ideal-gas EOS");
+    if (ncol < 1 || nx1 < 1 || nx1 + 2 * NG > IMPL_MAXNC1)
+      throw std::string("bad column count or length");
+    if (!w_cols || !du_cols || !area1 || !vol) throw std::string("null column array");
+    size_t nblk = ((size_t)ncol + 63) / 64;
+    if (nblk * nx1 * 15 * 64 * 2 > (size_t)EXO_NOALLOC_SCRATCH_DOUBLES)
+      throw std::string("columns need ") + std::to_string(nblk * nx1 * 15 * 64 * 2) +
+          " doubles of coefficient scratch, above the fixed " +
+          std::to_string(EXO_NOALLOC_SCRATCH_DOUBLES);
+    double *scr = scratch_ptr();
+    ColIOShim io{w_cols, du_cols, area1, vol, nx1 + 2 * NG, NG};
+    hipLaunchKernelGGL(k_implicit_columns, dim3((unsigned)nblk), dim3(64), 0, h->stream, io,
+                       ncol, nx1, dt, h->g.gamma, h->g.grav, scr);
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -269,6 +269,21 @@ EXO_API int exo_riemann_face(exo_handle *h, int dir, int p, int k, int j, const 
  * volumes; gamma and grav of this handle's configuration (ideal gas). */
 EXO_API int exo_implicit_column(exo_handle *h, int nx1, const double *w_col, double *du_col,
                                 const double *area1, const double *vol, double dt);
+/* No counterpart in the reference: exo_implicit_column for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  ncol columns at once, all arrays in
+ * device memory of the handle's GPU: column c's w and du at c*5*(nx1 + 6),
+ * its areas at c*(nx1 + 1), its volumes at c*nx1; du's interior levels are
+ * replaced by the correction.  The coefficient scratch is a fixed device
+ * array of EXO_NOALLOC_SCRATCH_DOUBLES doubles per GPU, 64-column blocks of
+ * nx1*1920 doubles each: larger requests are rejected (EXO_ERR_ARG).  Nothing
+ * is allocated and nothing synchronises: the work is queued on the handle's
+ * stream (exo_sync waits for it).  Calls that overlap on one GPU share the
+ * scratch and race. */
+#define EXO_NOALLOC_SCRATCH_DOUBLES (1 << 19)
+EXO_API int exo_implicit_columns_noalloc(exo_handle *h, int ncol, int nx1, const double *w_cols,
+                                         double *du_cols, const double *area1, const double *vol,
+                                         double dt);
 /* WENO5 value at the face between b and c seen from cell c (device). */
 EXO_API int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
 /* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y), or
