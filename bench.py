@@ -94,11 +94,11 @@ def step_compulsory_bytes_per_cell(nv, mixed):
 # kernel of the live profile -> its name in the rocprofv3 PMC summary, per form
 PMC_NAMES = {
     "fp64": {"implicit": "k_implicit<true>", "update": "k_update_na<false, 1>",
-             "flux_x1": "k_flux_x1", "flux_x2": "k_flux_h<1>", "flux_x3": "k_flux_h<2>",
+             "flux_x1": "k_flux_x1<4>", "flux_x2": "k_flux_h<1>", "flux_x3": "k_flux_h<2>",
              "edge_states": "k_edge_states", "ghost": "k_ghost",
              "prim": "k_prim", "dt": "k_dt"},
     "mixed": {"implicit": "k_implicit<true>", "update": "k_update_na<true, 1>",
-              "flux_x1": "k_flux_x1_mix", "flux_x2": "k_flux_h_mix<1, false>",
+              "flux_x1": "k_flux_x1_mix<4>", "flux_x2": "k_flux_h_mix<1, false>",
               "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
               "dt": "k_dt"},
 }
