@@ -3018,6 +3018,11 @@ void tune_launcher(Launcher &L, int reps) {
         L.cap[kind] = c;
         for (int r = -1; r < reps; ++r) {  // r = -1 warms up
           if (r == 0) HIPCHK(hipEventRecord(e0, L.stream));
+          // each candidate launch follows a primitive conversion, as within a
+          // step: its inputs come from memory just written, not from caches
+          // warmed by the previous repetition (the conversion's own time is
+          // the same for every candidate)
+          if (kind != K_PRIM) launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR);
           switch (kind) {
             case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
             case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
