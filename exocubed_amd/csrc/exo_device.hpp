@@ -24,6 +24,33 @@ template <typename R>
 EXO_HD R sqr(R x) { return x * x; }
 
 // ---------------------------------------------------------------------------
+// x / d in FP64 with the steps of gfx950's IEEE division (the compiler's
+// v_div_scale / v_rcp / Newton / v_div_fmas / v_div_fixup sequence) minus its
+// range scaling: the reciprocal refined by two Newton steps, the quotient, one
+// remainder correction (Markstein, IBM J. Res. Dev. 34(1), 1990), then
+// v_div_fixup for zeros, infinities and NaNs.  v_div_scale changes neither
+// operand, and v_div_fmas is a plain FMA, unless |x| < 2^-969, d is
+// subnormal, |d| >= 2^1022, |x / d| < 2^-1022 or x / d is within 2^-256 of
+// overflow; outside those ranges (none of which the hydro quantities reach)
+// the result is the IEEE quotient bit for bit, for 8 instructions instead of
+// 11 and without the VCC hand-off between v_div_scale and v_div_fmas.
+EXO_HD double qdiv(double x, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  double q = x * r;
+  double rem = fma(-d, q, x);
+  return __builtin_amdgcn_div_fixup(fma(rem, r, q), d, x);
+#else
+  return x / d;
+#endif
+}
+EXO_HD float qdiv(float x, float d) { return x / d; }
+
+// ---------------------------------------------------------------------------
 // Deterministic exp / log / pow (FP64) used by the hydrostatic decomposition
 // (change_to_perturbation.cpp:63-65,109-115,181-182; decomposition.cpp:357-360).
 // Cody-Waite reduction with a degree-13 Taylor polynomial for exp, and the
@@ -68,7 +95,7 @@ EXO_HD double dlog(double x) {
     m = m * 2.0;
     k = k - 1;
   }
-  double f = (m - 1.0) / (m + 1.0);
+  double f = qdiv(m - 1.0, m + 1.0);
   double s = f * f;
   double p = 1.0 / 25.0;
   p = 1.0 / 23.0 + s * p;
@@ -104,10 +131,10 @@ EXO_HD R weno5(R phim2, R phim1, R phi, R phip1, R phip2) {
   R b1 = R(13.) / R(12.) * sqr(phip1 - R(2.) * phi + phim1) + R(.25) * sqr(phip1 - phim1);
   R b2 = R(13.) / R(12.) * sqr(phip2 - R(2.) * phip1 + phi) +
          R(.25) * sqr(phip2 - R(4.) * phip1 + R(3.) * phi);
-  R a0 = R(.3) / sqr(b0 + R(1e-6));
-  R a1 = R(.6) / sqr(b1 + R(1e-6));
-  R a2 = R(.1) / sqr(b2 + R(1e-6));
-  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+  R a0 = qdiv(R(.3), sqr(b0 + R(1e-6)));
+  R a1 = qdiv(R(.6), sqr(b1 + R(1e-6)));
+  R a2 = qdiv(R(.1), sqr(b2 + R(1e-6)));
+  return qdiv(a0 * p0 + a1 * p1 + a2 * p2, a0 + a1 + a2);
 }
 
 // ---------------------------------------------------------------------------
@@ -166,15 +193,15 @@ EXO_HD void vec_rll_from_abp(double X, double Y, int blockID, double V2, double 
 template <typename R>
 EXO_HD void local_frame(int ivx, R cosf, R sinf, R *q) {
   // PrimToLocal2/3, gnomonic_equiangle.cpp:604-718
-  R g23 = cosf, gi = R(1.0) / (sinf * sinf);
+  R g23 = cosf, gi = qdiv(R(1.0), sinf * sinf);
   R uu1 = q[IVX], uu2 = q[IVY], uu3 = q[IVZ];
   if (ivx == IVY) {
-    R T22 = R(1.0) / sqrt(gi), T32 = g23 / sqrt(R(1.0)), T33 = sqrt(R(1.0));
+    R T22 = qdiv(R(1.0), R(sqrt(gi))), T32 = g23 / sqrt(R(1.0)), T33 = sqrt(R(1.0));
     q[IVX] = R(1.0) * uu1;
     q[IVY] = T22 * uu2;
     q[IVZ] = T32 * uu2 + T33 * uu3;
   } else {
-    R T22 = sqrt(R(1.0)), T23 = g23 / sqrt(R(1.0)), T33 = R(1.0) / sqrt(gi);
+    R T22 = sqrt(R(1.0)), T23 = g23 / sqrt(R(1.0)), T33 = qdiv(R(1.0), R(sqrt(gi)));
     q[IVX] = R(1.0) * uu1;
     q[IVY] = T22 * uu2 + T23 * uu3;
     q[IVZ] = T33 * uu3;
@@ -184,7 +211,7 @@ EXO_HD void local_frame(int ivx, R cosf, R sinf, R *q) {
 template <typename R>
 EXO_HD void flux_to_global(int ivx, R cosf, R sinf, R *f) {
   // FluxToGlobal2/3, gnomonic_equiangle.cpp:752-832
-  R g23 = cosf, gi = R(1.0) / (sinf * sinf);
+  R g23 = cosf, gi = qdiv(R(1.0), sinf * sinf);
   R txx = f[IVX], txy = f[IVY], txz = f[IVZ];
   if (ivx == IVY) {
     R T22 = sqrt(gi), T32 = -sqrt(gi) * g23 / R(1.0), T33 = R(1.0) / sqrt(R(1.0));
@@ -203,18 +230,18 @@ EXO_HD void flux_to_global(int ivx, R cosf, R sinf, R *f) {
 template <typename R>
 EXO_HD void roe_sw(int ivx, const R *wli, const R *wri, R *fl) {
   int ivy = ivx == IVY ? IVZ : IVY;
-  R ubar = (wli[ivx] * sqrt(wli[IDN]) + wri[ivx] * sqrt(wri[IDN])) /
-           (sqrt(wli[IDN]) + sqrt(wri[IDN]));
-  R vbar = (wli[ivy] * sqrt(wli[IDN]) + wri[ivy] * sqrt(wri[IDN])) /
-           (sqrt(wli[IDN]) + sqrt(wri[IDN]));
+  R ubar = qdiv(wli[ivx] * sqrt(wli[IDN]) + wri[ivx] * sqrt(wri[IDN]),
+               sqrt(wli[IDN]) + sqrt(wri[IDN]));
+  R vbar = qdiv(wli[ivy] * sqrt(wli[IDN]) + wri[ivy] * sqrt(wri[IDN]),
+               sqrt(wli[IDN]) + sqrt(wri[IDN]));
   R cbar = sqrt(R(0.5) * (wli[IDN] + wri[IDN]));
   R delh = wri[IDN] - wli[IDN];
   R delu = wri[ivx] - wli[ivx];
   R delv = wri[ivy] - wli[ivy];
   R hbar = sqrt(wli[IDN] * wri[IDN]);
-  R a1 = R(0.5) * (cbar * delh - hbar * delu) / cbar;
+  R a1 = qdiv(R(0.5) * (cbar * delh - hbar * delu), cbar);
   R a2 = hbar * delv;
-  R a3 = R(0.5) * (cbar * delh + hbar * delu) / cbar;
+  R a3 = qdiv(R(0.5) * (cbar * delh + hbar * delu), cbar);
   R w00 = a1, w01 = a1 * (ubar - cbar), w02 = a1 * vbar;
   R w12 = a2;
   R w20 = a3, w21 = a3 * (ubar + cbar), w22 = a3 * vbar;
@@ -246,7 +273,7 @@ EXO_HD void lmars_sw(int ivx, const R *wli, const R *wri, R *fl) {
   int ivy = ivx == IVY ? IVZ : IVY;
   R cbar = sqrt(R(0.5) * (wli[IDN] + wri[IDN]));
   R hbar = R(0.5) * (wli[IDN] + wri[IDN]) + R(0.5) * cbar * (wli[ivx] - wri[ivx]);
-  R ubar = R(0.5) * (wli[ivx] + wri[ivx]) + R(0.5) / cbar * (wli[IDN] - wri[IDN]);
+  R ubar = R(0.5) * (wli[ivx] + wri[ivx]) + qdiv(R(0.5), cbar) * (wli[IDN] - wri[IDN]);
   if (ubar > R(0.)) {
     fl[IDN] = ubar * wli[IDN];
     fl[ivx] = ubar * wli[IDN] * wli[ivx] + R(0.5) * hbar * hbar;
@@ -264,11 +291,11 @@ template <typename R>
 EXO_HD void hllc(int ivx, R gamma, const R *wli, const R *wri, R *fl, R cosc) {
   int ivy = IVX + ((ivx - IVX) + 1) % 3;
   int ivz = IVX + ((ivx - IVX) + 2) % 3;
-  R igm1 = R(1.0) / (gamma - R(1.0));
+  R igm1 = qdiv(R(1.0), gamma - R(1.0));
   R wl[5] = {wli[IDN], wli[ivx], wli[ivy], wli[ivz], wli[IPR]};
   R wr[5] = {wri[IDN], wri[ivx], wri[ivy], wri[ivz], wri[IPR]};
-  R cl = sqrt((R(1.) + (gamma - R(1)) * R(1.) / R(1.)) * wl[IPR] / wl[IDN]);
-  R cr = sqrt((R(1.) + (gamma - R(1)) * R(1.) / R(1.)) * wr[IPR] / wr[IDN]);
+  R cl = sqrt(qdiv((R(1.) + (gamma - R(1)) * R(1.) / R(1.)) * wl[IPR], wl[IDN]));
+  R cr = sqrt(qdiv((R(1.) + (gamma - R(1)) * R(1.) / R(1.)) * wr[IPR], wr[IDN]));
   R KE_l, KE_r;
   if (ivx == IVX) {
     R vy = wl[IVY] + wl[IVZ] * cosc, vz = wl[IVZ] + wl[IVY] * cosc;
@@ -287,10 +314,10 @@ EXO_HD void hllc(int ivx, R gamma, const R *wli, const R *wri, R *fl, R cosc) {
   R pmid = R(.5) * (wl[IPR] + wr[IPR] + (wl[IVX] - wr[IVX]) * rhoa * ca);
   R ql = (pmid <= wl[IPR])
              ? R(1.0)
-             : sqrt(R(1.0) + (gamma + R(1)) / (R(2) * gamma) * (pmid / wl[IPR] - R(1.0)));
+             : sqrt(R(1.0) + qdiv(gamma + R(1), R(2) * gamma) * (qdiv(pmid, wl[IPR]) - R(1.0)));
   R qr = (pmid <= wr[IPR])
              ? R(1.0)
-             : sqrt(R(1.0) + (gamma + R(1)) / (R(2) * gamma) * (pmid / wr[IPR] - R(1.0)));
+             : sqrt(R(1.0) + qdiv(gamma + R(1), R(2) * gamma) * (qdiv(pmid, wr[IPR]) - R(1.0)));
   R al = wl[IVX] - cl * ql;
   R ar = wr[IVX] + cr * qr;
   R bp = ar > R(0.0) ? ar : R(TINY_NUMBER);
@@ -300,8 +327,8 @@ EXO_HD void hllc(int ivx, R gamma, const R *wli, const R *wri, R *fl, R cosc) {
   R tr = wr[IPR] + vxr * wr[IDN] * wr[IVX];
   R ml = wl[IDN] * vxl;
   R mr = -(wr[IDN] * vxr);
-  R am = (tl - tr) / (ml + mr);
-  R cp = (ml * tr + mr * tl) / (ml + mr);
+  R am = qdiv(tl - tr, ml + mr);
+  R cp = qdiv(ml * tr + mr * tl, ml + mr);
   cp = cp > R(0.0) ? cp : R(0.0);
   vxl = wl[IVX] - bm;
   vxr = wr[IVX] - bp;
@@ -312,13 +339,13 @@ EXO_HD void hllc(int ivx, R gamma, const R *wli, const R *wri, R *fl, R cosc) {
   R fLe = el * vxl + wl[IPR] * wl[IVX], fRe = er * vxr + wr[IPR] * wr[IVX];
   R sl, sr, sm;
   if (am >= R(0.0)) {
-    sl = am / (am - bm);
+    sl = qdiv(am, am - bm);
     sr = R(0.0);
-    sm = -bm / (am - bm);
+    sm = qdiv(-bm, am - bm);
   } else {
     sl = R(0.0);
-    sr = -am / (bp - am);
-    sm = bp / (bp - am);
+    sr = qdiv(-am, bp - am);
+    sm = qdiv(bp, bp - am);
   }
   fl[IDN] = sl * fLd + sr * fRd;
   fl[ivx] = sl * fLx + sr * fRx + sm * cp;
@@ -332,17 +359,19 @@ template <typename R>
 EXO_HD void lmars(int ivx, R gamma, const R *wli, const R *wri, R *fl) {
   int ivy = IVX + ((ivx - IVX) + 1) % 3;
   int ivz = IVX + ((ivx - IVX) + 2) % 3;
-  R kappal = R(1.) / (gamma - R(1.)) * R(1.) / R(1.);
-  R kappar = R(1.) / (gamma - R(1.)) * R(1.) / R(1.);
-  R hl = wli[IPR] / wli[IDN] * (kappal + R(1.)) +
+  R kappal = qdiv(R(1.), gamma - R(1.)) * R(1.) / R(1.);
+  R kappar = qdiv(R(1.), gamma - R(1.)) * R(1.) / R(1.);
+  R hl = qdiv(wli[IPR], wli[IDN]) * (kappal + R(1.)) +
          R(0.5) * (sqr(wli[ivx]) + sqr(wli[ivy]) + sqr(wli[ivz]));
-  R hr = wri[IPR] / wri[IDN] * (kappar + R(1.)) +
+  R hr = qdiv(wri[IPR], wri[IDN]) * (kappar + R(1.)) +
          R(0.5) * (sqr(wri[ivx]) + sqr(wri[ivy]) + sqr(wri[ivz]));
   R rhobar = R(0.5) * (wli[IDN] + wri[IDN]);
-  R cbar = sqrt(R(0.5) * (R(1.) + (R(1.) / kappar + R(1.) / kappal) / R(2.)) *
-                (wli[IPR] + wri[IPR]) / rhobar);
+  R cbar = sqrt(qdiv(R(0.5) * (R(1.) + (qdiv(R(1.), kappar) + qdiv(R(1.), kappal)) / R(2.)) *
+                         (wli[IPR] + wri[IPR]),
+                     rhobar));
   R pbar = R(0.5) * (wli[IPR] + wri[IPR]) + R(0.5) * (rhobar * cbar) * (wli[ivx] - wri[ivx]);
-  R ubar = R(0.5) * (wli[ivx] + wri[ivx]) + R(0.5) / (rhobar * cbar) * (wli[IPR] - wri[IPR]);
+  R ubar =
+      R(0.5) * (wli[ivx] + wri[ivx]) + qdiv(R(0.5), rhobar * cbar) * (wli[IPR] - wri[IPR]);
   R rdl = R(1.), rdr = R(1.);
   if (ubar > R(0.)) {
     fl[IDN] = ubar * wli[IDN] * rdl;
@@ -469,7 +498,7 @@ EXO_HD M5 m5_inv(const M5 &A) {
         piv[r] = t;
       }
     }
-    double yc = 1.0 / lu[c][c];
+    double yc = qdiv(1.0, lu[c][c]);
     rd[c] = yc;
 #pragma unroll
     for (int r = c + 1; r < 5; ++r) {
@@ -569,7 +598,7 @@ EXO_HD M3 m3_inv(const M3 &A) {
         piv[r] = t;
       }
     }
-    double yc = 1.0 / lu[c][c];
+    double yc = qdiv(1.0, lu[c][c]);
     rd[c] = yc;
 #pragma unroll
     for (int r = c + 1; r < 3; ++r) {
@@ -604,16 +633,17 @@ EXO_HD M3 m3_inv(const M3 &A) {
 // flux_decomposition.hpp:34-61 RoeAverage (NVAPOR = 0)
 EXO_HD void roe_average(double *prim, double gm1, const double *wl, const double *wr) {
   double sqrtdl = sqrt(wl[IDN]), sqrtdr = sqrt(wr[IDN]);
-  double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+  double isdlpdr = qdiv(1.0, sqrtdl + sqrtdr);
   prim[IDN] = sqrtdl * sqrtdr;
   prim[IVX] = (sqrtdl * wl[IVX] + sqrtdr * wr[IVX]) * isdlpdr;
   prim[IVY] = (sqrtdl * wl[IVY] + sqrtdr * wr[IVY]) * isdlpdr;
   prim[IVZ] = (sqrtdl * wl[IVZ] + sqrtdr * wr[IVZ]) * isdlpdr;
-  double el = wl[IPR] / gm1 + 0.5 * wl[IDN] * (sqr(wl[IVX]) + sqr(wl[IVY]) + sqr(wl[IVZ]));
-  double er = wr[IPR] / gm1 + 0.5 * wr[IDN] * (sqr(wr[IVX]) + sqr(wr[IVY]) + sqr(wr[IVZ]));
-  double hbar = ((el + wl[IPR]) / sqrtdl + (er + wr[IPR]) / sqrtdr) * isdlpdr;
-  prim[IPR] = (hbar - 0.5 * (sqr(prim[IVX]) + sqr(prim[IVY]) + sqr(prim[IVZ]))) * gm1 /
-              (gm1 + 1.) * prim[IDN];
+  double el = qdiv(wl[IPR], gm1) + 0.5 * wl[IDN] * (sqr(wl[IVX]) + sqr(wl[IVY]) + sqr(wl[IVZ]));
+  double er = qdiv(wr[IPR], gm1) + 0.5 * wr[IDN] * (sqr(wr[IVX]) + sqr(wr[IVY]) + sqr(wr[IVZ]));
+  double hbar = (qdiv(el + wl[IPR], sqrtdl) + qdiv(er + wr[IPR], sqrtdr)) * isdlpdr;
+  prim[IPR] = qdiv((hbar - 0.5 * (sqr(prim[IVX]) + sqr(prim[IVY]) + sqr(prim[IVZ]))) * gm1,
+                   gm1 + 1.) *
+              prim[IDN];
 }
 
 // R |Lambda| R^-1 from Eigenvalue / Eigenvector (flux_decomposition.hpp:63-96).
@@ -624,7 +654,7 @@ EXO_HD void roe_average(double *prim, double gm1, const double *wl, const double
 EXO_HD M5 abs_a(const double *prim, double cs, double gm1) {
   double r = prim[IDN], u = prim[IVX], v = prim[IVY], w = prim[IVZ], p = prim[IPR];
   double ke = 0.5 * (u * u + v * v + w * w);
-  double hp = (gm1 + 1.) / gm1 * p / r;
+  double hp = qdiv(qdiv(gm1 + 1., gm1) * p, r);
   double h = hp + ke;
   double l0 = fabs(u - cs), l1 = fabs(u), l2 = fabs(u + cs), l3 = fabs(u), l4 = fabs(u);
   // columns 0..2 of R |Lambda|
@@ -636,7 +666,7 @@ EXO_HD M5 abs_a(const double *prim, double cs, double gm1) {
   // columns 3 and 4 of R |Lambda|: (R(2,3), R(4,3)) = (1, v), (R(3,4), R(4,4)) = (1, w)
   double r23 = 1. * l3, r43 = v * l3, r34 = 1. * l4, r44 = w * l4;
   double d1 = 2. * cs * hp, d2 = 2. * hp;
-  double y1 = 1. / d1, y2 = 1. / d2, y3 = 1. / hp;
+  double y1 = qdiv(1., d1), y2 = qdiv(1., d2), y3 = qdiv(1., hp);
   double Ri[3][5] = {{div_rn(cs * ke + u * hp, d1, y1), div_rn(-hp - cs * u, d1, y1),
                       div_rn(-v, d2, y2), div_rn(-w, d2, y2), y2},
                      {div_rn(hp - ke, hp, y3), div_rn(u, hp, y3), div_rn(v, hp, y3),

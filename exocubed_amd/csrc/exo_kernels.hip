@@ -239,9 +239,9 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
       st.u[g.c(IDN, p, k, i, j)] = ud;
     }
     double v = uv[IVY], ww = uv[IVZ];
-    double di = 1. / ud;
+    double di = qdiv(1., ud);
     // the four quotients by sth2 share one correctly rounded reciprocal
-    double sth2 = 1. - cth * cth, ys = 1. / sth2;
+    double sth2 = 1. - cth * cth, ys = qdiv(1., sth2);
     double wy = div_rn(v, sth2, ys) - div_rn(ww * cth, sth2, ys);
     double wz = div_rn(-v * cth, sth2, ys) + div_rn(ww, sth2, ys);
     rho_c = ud;
@@ -257,9 +257,9 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
     double density = 0.;
     density += u_d;
     rho_c = u_d;
-    double di = 1. / density;
+    double di = qdiv(1., density);
     double w_vx = u_m1 * di, w_vy = u_m2 * di, w_vz = u_m3 * di;
-    double sth2 = 1. - cth * cth, ys = 1. / sth2;
+    double sth2 = 1. - cth * cth, ys = qdiv(1., sth2);
     double v = w_vy, ww = w_vz;
     w_vy = div_rn(v, sth2, ys) - div_rn(ww * cth, sth2, ys);
     w_vz = div_rn(-v * cth, sth2, ys) + div_rn(ww, sth2, ys);
@@ -392,17 +392,17 @@ __device__ __forceinline__ void ghost_body(const DGrid &g, int lb,
   for (int n = 0; n < nvar; ++n) {
     if (vectors && (n == IVY || n == IVZ || g.skipv(n))) continue;
     double v1 = arr[n * g.vstride + o0], v2 = arr[n * g.vstride + o1];
-    arr[n * g.vstride + ot] = (d.A * v1 + d.B * v2) / d.D;
+    arr[n * g.vstride + ot] = qdiv(d.A * v1 + d.B * v2, d.D);
   }
   if (vectors) {
     double v1y = arr[IVY * g.vstride + o0], v1z = arr[IVZ * g.vstride + o0];
     double v2y = arr[IVY * g.vstride + o1], v2z = arr[IVZ * g.vstride + o1];
-    double vy = (d.A * v1y + d.B * v2y) / d.D;
-    double vz = (d.A * v1z + d.B * v2z) / d.D;
+    double vy = qdiv(d.A * v1y + d.B * v2y, d.D);
+    double vz = qdiv(d.A * v1z + d.B * v2z, d.D);
     double sth = d.sth_s, cth = d.cth_s;
     double vyt = vy, vzt = vz;
-    vy = vyt / sth / sth - vzt * cth / sth / sth;
-    vz = vzt / sth / sth - vyt * cth / sth / sth;
+    vy = qdiv(qdiv(vyt, sth), sth) - qdiv(qdiv(vzt * cth, sth), sth);
+    vz = qdiv(qdiv(vzt, sth), sth) - qdiv(qdiv(vyt * cth, sth), sth);
     double U, V;
     vec_rll_from_abp(d.src_cy, d.src_cz, d.src_block, vy, vz, &U, &V);
     vec_abp_from_rll(d.tgt_cy, d.tgt_cz, d.tgt_block, U, V, &vy, &vz);
@@ -445,8 +445,8 @@ __device__ __forceinline__ void psf_scan(const DGrid &g, const DState &st, int p
   int is = g.is, ie = g.ie;
   double grav = -g.grav;
   double pie = st.w[g.c(IPR, p, k, ie, j)], die = st.w[g.c(IDN, p, k, ie, j)];
-  double RdTv = pie / die;
-  PSF(ie + 1) = pie * dexp(-grav * g.dx1f[ie] / (2 * RdTv));
+  double RdTv = qdiv(pie, die);
+  PSF(ie + 1) = pie * dexp(qdiv(-grav * g.dx1f[ie], 2 * RdTv));
   // the densities of eight levels are loaded before the recurrence uses them
   constexpr int SB = 8;
   for (int i0 = ie; i0 >= is; i0 -= SB) {
@@ -485,8 +485,8 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
   if (decomp) {
     double ent0 = g.gamma;
     double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
-    ig = 1. / ent0;
-    e1 = dexp(-ent1 / ent0);
+    ig = qdiv(1., ent0);
+    e1 = dexp(qdiv(-ent1, ent0));
   }
   // this wave's faces [f0, f1)
   int nf = ie + 2 - is;
@@ -499,7 +499,7 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
     if (decomp) {
       double a = PSF(i), b = PSF(i + 1), psv;
       if (fabs(a - b) < 1.E-6) psv = (a + b) / 2.;
-      else psv = (a - b) / dlog(a / b);
+      else psv = qdiv(a - b, dlog(qdiv(a, b)));
       double dsv = dpow(psv, ig) * e1;
       q[slot][IPR] -= psv;
       q[slot][IDN] -= dsv;
@@ -826,7 +826,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   double vol = g.volume(p, k, j, i);
   // every quotient by vol (and by the radius below) shares one correctly
   // rounded reciprocal: div_rn gives the IEEE quotient
-  double yvol = 1. / vol;
+  double yvol = qdiv(1., vol);
   // the first stage only sets u1 = delta u (u itself is unchanged when delta
   // is 1), later stages keep u1 unless delta != 0: arrays that would be read
   // only to be overwritten, or rewritten unchanged, are not touched
@@ -893,7 +893,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   double v1 = g.eos == EOS_SW ? 0. : st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
   double v3 = st.w[g.c(IVZ, p, k, i, j)];
   double rho = st.w[g.c(IDN, p, k, i, j)];
-  double radius = g.x1v[i], yrad = 1. / radius;
+  double radius = g.x1v[i], yrad = qdiv(1., radius);
   double v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
   double pr;
   if (g.eos == EOS_SW) {
@@ -932,7 +932,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
     vec_abp_from_rll(X, Y, p + 1, -llU, llV, &acc2, &acc3);  // GetVyVz
     double C = sqrt(X * X + 1), D = sqrt(Y * Y + 1);
-    double cth = -X * Y / (C * D);
+    double cth = qdiv(-X * Y, C * D);
     double a2c = acc2 + acc3 * cth, a3c = acc3 + acc2 * cth;
     if (g.coriolis == 1) {
       out[IVY] += wght * rho * a2c;
@@ -1183,8 +1183,8 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
   };
   double gm1c = (gamma - 1.) * 1. / 1.;
   double gm1 = 0.5 * (gm1c + gm1c);
-  double k1 = (gm1c + 1) / gm1c;
-  auto erow = [&](const double *w) { return k1 * w[IPR] / w[IDN]; };
+  double k1 = qdiv(gm1c + 1, gm1c);
+  auto erow = [&](const double *w) { return qdiv(k1 * w[IPR], w[IDN]); };
   // |A| at face is-1/2: the reference's row is-1 (full_correction.cpp:272-325)
   // leaves Am = R|L|R^-1 of the Roe state of (w(is-1), w(is))
   double wm[5], w0[5], wp[5], prim[5];
@@ -1193,11 +1193,11 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
   io.wraw(is, w0);
   io.project(w0);
   roe_average(prim, gm1, wm, w0);
-  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  double cs = sqrt(qdiv((1. + (gamma - 1) * 1. / 1.) * prim[IPR], prim[IDN]));
   M5 Am = abs_a(prim, cs, gm1);
   double em = erow(wm), e0 = erow(w0);
   // divisions by dt and by 2 vol share one correctly rounded reciprocal each
-  double ydt = 1. / dt;
+  double ydt = qdiv(1., dt);
   M5 Aprev;
   V5 dprev;
   // The next level's state and tendency are loaded one level ahead and put
@@ -1230,7 +1230,7 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
     }
     double aleft, aright, v;
     io.geom(i, aleft, aright, v);
-    double v2 = 2. * v, yv2 = 1. / v2;
+    double v2 = 2. * v, yv2 = qdiv(1., v2);
     // b, then b Aprev and b dprev, so b dies early (the first level keeps it
     // for the bottom fold)
     M5 b;
@@ -1249,7 +1249,7 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
       bd = m5_mulv(b, dprev);
     }
     roe_average(prim, gm1, w0, wp);
-    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    cs = sqrt(qdiv((1. + (gamma - 1) * 1. / 1.) * prim[IPR], prim[IDN]));
     M5 Ap = abs_a(prim, cs, gm1);
     auto cmat = [&]() {
       M5 c;
@@ -1498,18 +1498,18 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   };
   double gm1c = (gamma - 1.) * 1. / 1.;
   double gm1 = 0.5 * (gm1c + gm1c);
-  double k1 = (gm1c + 1) / gm1c;
-  auto erow = [&](const double *w) { return k1 * w[IPR] / w[IDN]; };
+  double k1 = qdiv(gm1c + 1, gm1c);
+  auto erow = [&](const double *w) { return qdiv(k1 * w[IPR], w[IDN]); };
   double wm[5], w0[5], wp[5], prim[5];
   wload(is - 1, wm);
   wproj(wm);
   wload(is, w0);
   wproj(w0);
   roe_average(prim, gm1, wm, w0);
-  double cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+  double cs = sqrt(qdiv((1. + (gamma - 1) * 1. / 1.) * prim[IPR], prim[IDN]));
   M5 Am = abs_a(prim, cs, gm1);
   double em = erow(wm), e0 = erow(w0);
-  double ydt = 1. / dt;
+  double ydt = qdiv(1., dt);
   M3 Aprev;
   V3 dprev;
   // the next level's state and tendency are loaded a level ahead and put to
@@ -1535,9 +1535,9 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
     double aleft = ((a23 * x1f[i]) * x1f[i]) * sth;
     double aright = ((a23 * x1f[i + 1]) * x1f[i + 1]) * sth;
     double v = ((((dx1f[i] * dx2a) * dx3a) * x1v[i]) * x1v[i]) * sth;
-    double v2 = 2. * v, yv2 = 1. / v2;
+    double v2 = 2. * v, yv2 = qdiv(1., v2);
     roe_average(prim, gm1, w0, wp);
-    cs = sqrt((1. + (gamma - 1) * 1. / 1.) * prim[IPR] / prim[IDN]);
+    cs = sqrt(qdiv((1. + (gamma - 1) * 1. / 1.) * prim[IPR], prim[IDN]));
     M5 Ap = abs_a(prim, cs, gm1);
     M5 Jm = jac_e(gm1c, wm, em), J0 = jac_e(gm1c, w0, e0), Jp = jac_e(gm1c, wp, ep);
     M3 b, c, T;
@@ -1757,14 +1757,14 @@ __device__ __forceinline__ void dt_body(const DGrid &g, const DState &st, int lb
     for (int n = 0; n < 5; ++n)
       if (!g.skipv(n)) wi[n] = st.w[g.c(n, p, k, i, j)];
     double cs = g.eos == EOS_SW ? sqrt(wi[IDN])
-                                : sqrt((1. + (g.gamma - 1) * 1. / 1.) * wi[IPR] / wi[IDN]);
+                                : sqrt(qdiv((1. + (g.gamma - 1) * 1. / 1.) * wi[IPR], wi[IDN]));
     double dt1 = g.dx1f[i];
     double dt2 = g.x1v[i] * g.T(T_DX2A, p, k, j), dt3 = g.x1v[i] * g.T(T_DX3A, p, k, j);
     double sp1 = fabs(wi[IVX]) + cs, sp2 = fabs(wi[IVY]) + cs, sp3 = fabs(wi[IVZ]) + cs;
-    if ((g.implicit & 1) && N > 1) dt1 /= g.cfl * fabs(wi[IVX]);
-    else dt1 /= sp1;
-    dt2 /= sp2;
-    dt3 /= sp3;
+    if ((g.implicit & 1) && N > 1) dt1 = qdiv(dt1, g.cfl * fabs(wi[IVX]));
+    else dt1 = qdiv(dt1, sp1);
+    dt2 = qdiv(dt2, sp2);
+    dt3 = qdiv(dt3, sp3);
     if (g.cfl_x1 && dt1 < m) m = dt1;
     if (dt2 < m) m = dt2;
     if (dt3 < m) m = dt3;
@@ -1991,19 +1991,19 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
   int f0 = is + (nf * seg) / W, f1 = is + (nf * (seg + 1)) / W;
   double ent0 = g.gamma;
   double ent1 = dlog(st.w[g.c(IPR, p, k, is, j)]) - ent0 * dlog(st.w[g.c(IDN, p, k, is, j)]);
-  double ig = 1. / ent0, e1 = dexp(-ent1 / ent0);
+  double ig = qdiv(1., ent0), e1 = dexp(qdiv(-ent1, ent0));
   // cell i from rho_sf at its lower face: psv, rho_sv and rho_sf at face i+1
   auto decomp = [&](int i, double rsf_lo, double *psv, double *rsv) -> double {
     double a = PSF(i), b = PSF(i + 1);
-    double t = (a - b) / (a + b);
+    double t = qdiv(a - b, a + b);
     if (fabs(t) > 0.3) {  // outside the series' range: the exact functions
-      double ps = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / dlog(a / b);
+      double ps = fabs(a - b) < 1.E-6 ? (a + b) / 2. : qdiv(a - b, dlog(qdiv(a, b)));
       *psv = ps;
       *rsv = dpow(ps, ig) * e1;
       return dpow(b, ig) * e1;
     }
     double lam = 2.0 * t * atanh_over_t(t * t);
-    *psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : (a - b) / lam;
+    *psv = fabs(a - b) < 1.E-6 ? (a + b) / 2. : qdiv(a - b, lam);
     *rsv = rsf_lo * exp_small(ig * ln_expm1_ratio(lam));
     return rsf_lo * exp_small(-ig * lam);
   };
@@ -2335,9 +2335,9 @@ __global__ void k_weno_batch(const double *in5, double *out, int n) {
 __global__ void k_math_batch(int op, const double *x, double y, double *out, int n) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  if (op == 3) {
+  if (op == 3 || op == 4) {
     double a = x[2 * t], d = x[2 * t + 1];
-    out[t] = div_rn(a, d, 1.0 / d);
+    out[t] = op == 3 ? div_rn(a, d, 1.0 / d) : qdiv(a, d);
     return;
   }
   out[t] = op == 0 ? dexp(x[t]) : (op == 1 ? dlog(x[t]) : dpow(x[t], y));
@@ -3855,8 +3855,8 @@ int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n) {
 int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n) {
   return guarded(h, [&] {
     if (n <= 0) return;
-    if (op < 0 || op > 3) throw std::string("bad op");
-    size_t nin = (size_t)n * (op == 3 ? 2 : 1);
+    if (op < 0 || op > 4) throw std::string("bad op");
+    size_t nin = (size_t)n * (op >= 3 ? 2 : 1);
     double *d;
     HIPCHK(hipMalloc(&d, (nin + n) * sizeof(double)));
     HIPCHK(hipMemcpyAsync(d, x, nin * sizeof(double), hipMemcpyHostToDevice, h->stream));

@@ -286,8 +286,10 @@ EXO_API int exo_implicit_columns_noalloc(exo_handle *h, int ncol, int nx1, const
                                          double dt);
 /* WENO5 value at the face between b and c seen from cell c (device). */
 EXO_API int exo_weno5_batch(exo_handle *h, const double *in5, double *out, int n);
-/* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y), or
- * (op 3) the shared-reciprocal division x[2t] / x[2t+1] the implicit solve uses */
+/* deterministic device exp/log/pow on n inputs (op 0 exp, 1 log, 2 pow y),
+ * (op 3) the shared-reciprocal division x[2t] / x[2t+1] the implicit solve
+ * uses, or (op 4) the device's unscaled IEEE division sequence of x[2t] / x[2t+1]
+ * that the FP64 kernels use for every quotient */
 EXO_API int exo_math_batch(exo_handle *h, int op, const double *x, double y, double *out, int n);
 
 /* Host-only introspection of the set-up tables (no GPU needed): which 0 ghost

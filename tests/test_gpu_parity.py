@@ -71,6 +71,35 @@ def test_device_shared_reciprocal_division_is_ieee(ex):
     assert np.array_equal(out, x / d)
 
 
+def test_device_division_is_ieee(ex):
+    """the FP64 kernels' quotient (reciprocal, two Newton steps, one remainder
+    correction, v_div_fixup; no range scaling) equals numpy's IEEE x / d bit for
+    bit: exponents over +-300, hard significands, signed zeros, infinities and
+    NaN in either operand"""
+    rng = np.random.default_rng(11)
+    n = 600000
+    x = rng.standard_normal(n) * np.exp2(rng.integers(-300, 301, n))
+    d = rng.standard_normal(n) * np.exp2(rng.integers(-300, 301, n))
+    hard = np.array([1.0, np.nextafter(2.0, 0.0), np.nextafter(1.0, 2.0), 1.5,
+                     np.nextafter(1.0, 0.0), 3.0, 1.0 + 2.0 ** -52 * 3, 1.0 / 3.0])
+    d[::5] = rng.choice(hard, len(d[::5])) * np.exp2(rng.integers(-40, 41, len(d[::5])))
+    x[1::5] = rng.choice(hard, len(x[1::5])) * np.exp2(rng.integers(-40, 41, len(x[1::5])))
+    x[2::5] = d[2::5] * rng.choice(hard, len(x[2::5]))  # quotients near 1
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -2.5, 1e-250, 1e250])
+    sx, sd = np.meshgrid(special, special)
+    x[:sx.size], d[:sx.size] = sx.ravel(), sd.ravel()
+    pairs = np.ascontiguousarray(np.stack([x, d], axis=1).ravel())
+    g = ex.Solver(sw_config(ex))
+    out = np.zeros(n)
+    assert g.L.exo_math_batch(g.h, 4, pairs, 0.0, out, n) == 0
+    g.close()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = x / d
+    nan = np.isnan(ref)
+    assert np.array_equal(np.isnan(out), nan)
+    assert np.array_equal(out[~nan].view(np.uint64), ref[~nan].view(np.uint64))
+
+
 @pytest.mark.parametrize("rsolver", [0, 1])
 def test_shallow_water_fluxes(ex, orc, rsolver):
     cfg = sw_config(ex, nx=8, rsolver=rsolver)
