@@ -810,9 +810,13 @@ __device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u
 // NOSCAL: the instance has no passive scalars (the caller checked nt == 0),
 // so the cell's update stores nothing and a caller can form several cells'
 // updates with shared loads.
-template <bool MIX, bool NOAVG = false, bool NOSCAL = false>
+// X2R (FP64 only): the cell's two x2 face fluxes come from the caller's
+// registers (x2l: face j, x2r: face j + 1) instead of the flux array.
+template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
-                                            int j, StageW sw, double *out) {
+                                            int j, StageW sw, double *out,
+                                            const double *x2l = nullptr,
+                                            const double *x2r = nullptr) {
   double dt = st.dtv[0];
   double wght = sw.beta * dt;
   bool tend = g.eos == EOS_IDEAL;
@@ -857,7 +861,9 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       if (g.eos == EOS_SW) dflx = 0.;
       else
         dflx = a1r * st.flx[0][g.f(n, p, k, i + 1, j)] - a1l * st.flx[0][g.f(n, p, k, i, j)];
-      dflx += a2r * st.flx[1][g.f(n, p, k, i, j + 1)] - a2l * st.flx[1][g.f(n, p, k, i, j)];
+      double f2r = X2R ? x2r[n] : st.flx[1][g.f(n, p, k, i, j + 1)];
+      double f2l = X2R ? x2l[n] : st.flx[1][g.f(n, p, k, i, j)];
+      dflx += a2r * f2r - a2l * f2l;
       dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
     } else {
       // FP32 perturbation fluxes plus the FP64 reference-pressure parts (the
@@ -1055,6 +1061,52 @@ __global__ void __launch_bounds__(BLK) k_update_na(const Inst *__restrict__ I_, 
   WORK_LOOP(w) {
     WORK_INST(w);
     update_na_body<MIX, LV>(g, st, lb, sw);
+  }
+}
+
+// The x2 faces and the stage update (tendency only) of an FP64 ideal gas
+// without passive scalars in one pass, so the x2 face fluxes never go through
+// memory.  The faces of the instance are numbered row by row (rows (p, k, i)
+// of N + 1 faces, j fastest); wavefront w takes faces 63 w .. 63 w + 63, lane
+// l face 63 w + l.  Lanes 0..62 then update the cell right of their face (its
+// face j), with face j + 1 from the next lane; a row's last face updates no
+// cell.  Every face and every cell is covered once, with the arithmetic of
+// face_flux64<1> and update_cell.
+constexpr int F2_FACES = 63;  // cells a wavefront updates (faces per wavefront - 1)
+
+__device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &st, int lb,
+                                                  StageW sw) {
+  if (lb == 0 && threadIdx.x == 0) *st.bad = 0;  // as update_na_body
+  int N = g.N, ni = g.nx1, na = N + 1;
+  size_t total = (size_t)6 * N * ni * na;
+  int lane = threadIdx.x & 63;
+  size_t F = ((size_t)lb * (BLK / 64) + (threadIdx.x >> 6)) * F2_FACES + lane;
+  bool act = F < total;
+  size_t r = act ? F : total - 1;
+  int fa = (int)(r % na);
+  r /= na;
+  int i = g.is + (int)(r % ni);
+  r /= ni;
+  int kk = (int)(r % N), p = (int)(r / N);
+  int k = g.ks + kk, j = g.js + fa;
+  double fl[5] = {0., 0., 0., 0., 0.}, fr[5];
+  if (act) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
+  // every lane takes part in the exchange
+#pragma unroll
+  for (int n = 0; n < 5; ++n) fr[n] = __shfl_down(fl[n], 1);
+  if (act && lane < F2_FACES && fa < N) {
+    double out[5];
+    update_cell<false, true, true, true>(g, st, p, k, i, j, sw, out, fl, fr);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i, j)] = out[n];
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_face2_update(const Inst *__restrict__ I_, Work w,
+                                                      StageW sw) {
+  WORK_LOOP(w) {
+    WORK_INST(w);
+    face2_update_body(g, st, lb, sw);
   }
 }
 
@@ -2439,7 +2491,11 @@ enum {
   // within a stage, instances whose full column solve forms the stage's
   // weighted average: their update (tendency only) and their solve; the
   // update of every other instance
-  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG, K_NKIND
+  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG,
+  // within a stage, the FP64 ideal gas without scalars takes its x2 faces and
+  // update in one kernel (k_face2_update); every other instance its separate
+  // x2 face kernel and update
+  K_FUSE2, K_FACE_NF, K_UPD_NA_NF, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2578,6 +2634,16 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, int shape
     case K_IMPL3_AVG:
       return ideal && (g.implicit & 1) && !(g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
     case K_ADD_DU_AVG: return ideal && !(g.implicit & 1) ? (int)grid1(cells) : 0;
+    case K_FUSE2: {
+      if (!ideal || mixed || g.nt) return 0;
+      size_t waves = (faces + F2_FACES - 1) / F2_FACES;
+      return (int)((waves + BLK / 64 - 1) / (BLK / 64));
+    }
+    case K_FACE_NF: return !mixed && !(ideal && !g.nt) ? (int)grid1(faces) : 0;
+    case K_UPD_NA_NF: {
+      int lv = shape ? shape : UPD_NA_L;
+      return ideal && !mixed && g.nt ? (int)grid1(cells / g.nx1 * ((g.nx1 + lv - 1) / lv)) : 0;
+    }
     // implicit_flag bit 1 solves the x1 columns, bit 8 with the full 5x5
     // system (solve_implicit_3d.cpp:102-135); without bit 1, u += du
     case K_IMPL: return ideal && (g.implicit & 1) && (g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
@@ -2653,13 +2719,14 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
 // a kind's block shape; the block table of a kind whose block count depends
 // on it (the stage updates) is rebuilt on the device
 void set_shape(Launcher &L, int kind, int shape) {
-  bool x1 = kind == K_X1 || kind == K_X1_MIX, upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
+  bool x1 = kind == K_X1 || kind == K_X1_MIX,
+       upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA || kind == K_UPD_NA_NF;
   bool ok = shape == 0 || (x1 && (shape == 2 || shape == 4 || shape == 8)) ||
             (upd && (shape == 1 || shape == 2 || shape == 4));
   if (!ok) throw std::string("no block shape ") + std::to_string(shape) + " for launch kind " +
                std::to_string(kind);
   L.shape[kind] = shape;
-  if (kind != K_UPD_NA && kind != K_UPD_MIX_NA) return;
+  if (!upd) return;
   int n = L.ninst;
   std::vector<int> row(n + 1);
   long long acc = 0;
@@ -2868,13 +2935,15 @@ void launch_x1(Launcher &L, int kind, int kid) {
   }
 }
 
-void launch_fluxes(Launcher &L) {
+// fused: within a stage, the x2 faces of the instances k_face2_update covers
+// are left to it
+void launch_fluxes(Launcher &L, bool fused = false) {
   launch_x1(L, K_X1, KID_FLUX_X1);
   launch_x1(L, K_X1_MIX, KID_FLUX_X1);
   launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
   launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
   launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
-  launch(L, K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
+  launch(L, fused ? K_FACE_NF : K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
   launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1, false>, BLK, 0);
   launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2, false>, BLK, 0);
@@ -2901,7 +2970,7 @@ void launch_update(Launcher &L, int stage, int prim) {
 
 // the ideal gas's stage update (tendency only) with its tuned levels per thread
 void launch_update_na(Launcher &L, int kind, int kid, StageW sw) {
-  bool mix = kind == K_UPD_MIX_NA;
+  bool mix = kind == K_UPD_MIX_NA;  // K_UPD_NA and K_UPD_NA_NF are FP64
   switch (L.shape[kind] ? L.shape[kind] : UPD_NA_L) {
     case 2:
       if (mix) launch(L, kind, kid, k_update_na<true, 2>, BLK, 0, sw);
@@ -2918,13 +2987,19 @@ void launch_update_na(Launcher &L, int kind, int kid, StageW sw) {
 }
 
 // the stage's update and column correction; the instances with the full
-// column solve leave the weighted average of u to it
-void launch_update_implicit(Launcher &L, int stage) {
+// column solve leave the weighted average of u to it.  fused: the x2 faces
+// of the instances k_face2_update covers were left to it (launch_fluxes)
+void launch_update_implicit(Launcher &L, int stage, bool fused = false) {
   int st0 = stage - 1;
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
             1, DFLOOR, SFLOOR};
   launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
-  launch_update_na(L, K_UPD_NA, KID_UPDATE, sw);
+  if (fused) {
+    launch(L, K_FUSE2, KID_UPDATE, k_face2_update, BLK, 0, sw);
+    launch_update_na(L, K_UPD_NA_NF, KID_UPDATE, sw);
+  } else {
+    launch_update_na(L, K_UPD_NA, KID_UPDATE, sw);
+  }
   launch_update_na(L, K_UPD_MIX_NA, KID_UPDATE, sw);
   launch(L, K_IMPL_AVG, KID_IMPLICIT, k_implicit<true>, 64, 0, sw);
   launch(L, K_IMPL3_AVG, KID_IMPLICIT, k_implicit3<true>, 64, 0, sw);
@@ -2961,8 +3036,8 @@ void launch_fill_ghosts_stage(Launcher &L) {
 }
 
 void run_stage(Launcher &L, int stage) {
-  launch_fluxes(L);
-  launch_update_implicit(L, stage);
+  launch_fluxes(L, true);
+  launch_update_implicit(L, stage, true);
   launch_fill_ghosts_stage(L);
 }
 
@@ -2999,7 +3074,8 @@ void tune_launcher(Launcher &L, int reps) {
   launch_fluxes(L);
   StageW sw{1.0, 0.0, 1.0, 1.0, 1, 0, DFLOOR, SFLOOR};
   const int kinds[] = {K_PRIM,     K_GHOST_U, K_X1,          K_X1_MIX, K_EDGE,   K_EDGE_MIX,
-                       K_FACE,     K_FACE_MIX, K_FACE_MIX_SW, K_DT,     K_UPD_NA, K_UPD_MIX_NA};
+                       K_FACE,     K_FACE_MIX, K_FACE_MIX_SW, K_DT,     K_UPD_NA, K_UPD_MIX_NA,
+                       K_FUSE2};
   for (int kind : kinds) {
     bool upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
     if (upd && scalars) continue;
@@ -3035,6 +3111,7 @@ void tune_launcher(Launcher &L, int reps) {
             case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
             case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
             case K_UPD_NA: case K_UPD_MIX_NA: launch_update_na(L, kind, -1, sw); break;
+            case K_FUSE2: launch(L, K_FUSE2, -1, k_face2_update, BLK, 0, sw); break;
           }
         }
         HIPCHK(hipEventRecord(e1, L.stream));
@@ -3055,6 +3132,7 @@ void tune_launcher(Launcher &L, int reps) {
   // the plain launch: they stream and cannot be repeated without changing the
   // state)
   L.cap[K_GHOST_S] = L.cap[K_GHOST_U];
+  L.cap[K_FACE_NF] = L.cap[K_FACE];
   L.prof = prof;
   HIPCHK(hipEventDestroy(e0));
   HIPCHK(hipEventDestroy(e1));

@@ -163,7 +163,7 @@ EXO_API int exo_conserved_sums(exo_handle *h, double *out5);
  * be NULL) receives up to ncaps values: the width of each launch kind (0 = one
  * workgroup per block), then, from index EXO_NKINDS on, its shape (0 = the
  * default). */
-#define EXO_NKINDS 27
+#define EXO_NKINDS 30
 EXO_API int exo_tune(exo_handle *h, int reps, int *caps, int ncaps);
 
 /* Device bytes exo_create allocates for this configuration (host only; 0 for
