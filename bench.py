@@ -123,19 +123,31 @@ def traffic_profile():
         return None
 
 
-def measured_traffic(kernel, form):
+def pmc_name(kernel, form, tuned=None):
+    """the kernel's name in the counter summary; the x1 sweep's template
+    argument is the autotuned waves per block (tuned["x1_shape"] or
+    tuned["x1_mix_shape"], default 4)"""
+    name = PMC_NAMES[form].get(kernel)
+    if name and kernel == "flux_x1" and tuned:
+        sh = tuned.get("x1_mix_shape" if form == "mixed" else "x1_shape")
+        if sh:
+            name = name.replace("<4>", f"<{sh}>")
+    return name
+
+
+def measured_traffic(kernel, form, tuned=None):
     """HBM bytes per launch of one kernel from the committed counter passes"""
     t = traffic_profile()
     if not t:
         return None
-    k = t["kernels"].get(PMC_NAMES[form].get(kernel, ""))
+    k = t["kernels"].get(pmc_name(kernel, form, tuned) or "")
     return k["hbm_bytes"] if k else None
 
 
-def step_traffic_per_cell(form, cells):
+def step_traffic_per_cell(form, cells, tuned=None):
     """Measured HBM bytes per cell and step from the counter passes: the sum
-    over one step's launches (STEP_LAUNCHES) of the form's kernels; None if
-    the profile lacks one of them"""
+    over one step's launches (STEP_LAUNCHES) of the form's kernels (with the
+    run's autotuned x1 block shape); None if the profile lacks one of them"""
     t = traffic_profile()
     if not t:
         return None
@@ -143,7 +155,7 @@ def step_traffic_per_cell(form, cells):
     for k, n in STEP_LAUNCHES.items():
         if k not in PMC_NAMES[form]:
             continue
-        b = t["kernels"].get(PMC_NAMES[form][k])
+        b = t["kernels"].get(pmc_name(k, form, tuned))
         if b is None:
             return None
         tot += n * b["hbm_bytes"]
@@ -632,7 +644,7 @@ def main():
     bpc = IMPLICIT_SURVEY_BYTES if dom == "implicit" else own
     achieved = (bpc * r["cells"] / avg_s / 1e9) if bpc else None
     own_gbs = (own * r["cells"] / avg_s / 1e9) if own else None
-    traffic = measured_traffic(dom, form) if iflag == 9 else None
+    traffic = measured_traffic(dom, form, r["tuned"]) if iflag == 9 else None
     tp = traffic_profile()
     roof = {"bound": "hbm", "kernel": dom, "form": form, "achieved": achieved,
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -691,7 +703,7 @@ def main():
     recov = None if args.only_headline else recovery_bench(ex)
     part = None if args.only_headline else partial_correction(ex, min(args.steps, 50),
                                                                args.warmup)
-    step_meas = step_traffic_per_cell(form, r["cells"]) if iflag == 9 else None
+    step_meas = step_traffic_per_cell(form, r["cells"], r["tuned"]) if iflag == 9 else None
     out = {
         "metric": "cell-updates/s mixed vs FP64 on 1 MI355X at <=1e-5 rel L2; p50/p99 run latency",
         "value": value,
