@@ -52,7 +52,7 @@ def workload_config(ex, precision, implicit_flag=9):
 IC_PARAMS = (250.0, 1.0e5, 287.0, 0.01, 5.0)
 
 
-def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
+def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0, implicit_flag=9):
     """Compulsory HBM bytes per interior cell and launch of each kernel: every
     array it must read or write once (state FP64; face fluxes FP32 in the
     mixed ideal gas, which also stores the FP64 x1 face pressure and whose
@@ -70,9 +70,11 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0):
         # x2 faces in the same kernel (k_face2_update): read w, F1, F3, write du
         "update": (4 * nv * 8 if not mixed and nt == 0 else
                    2 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f)),
-        # k_implicit<AVG>: read du, w, u, u1 (stages 2-3) and write u, or read
-        # du, w, u and write u, u1 (stage 1)
-        "implicit": 5 * nv * 8,
+        # the column step of the stage (KID_IMPLICIT): k_implicit<true> (5x5,
+        # implicit_flag 9) and k_implicit3<true> (3x3, implicit_flag 1) read du,
+        # w, u, u1 (stages 2-3) and write u, or read du, w, u and write u, u1
+        # (stage 1); k_add_du<true> (no solve) reads no w: 4 V s
+        "implicit": 5 * nv * 8 if implicit_flag & 1 else 4 * nv * 8,
         "prim": 2 * nv * 8 + 2 * nt * 8 + ((nv + nt) * 4 if mixed else 0),  # read u, s;
         # write w, r (mixed: and fw)
         "dt": nv * 8,               # read w
@@ -162,14 +164,14 @@ def step_traffic_per_cell(form, cells, tuned=None):
     return tot / cells
 
 
-def kernel_rooflines(prof, cells, nv, mixed, nt=0):
+def kernel_rooflines(prof, cells, nv, mixed, nt=0, implicit_flag=9):
     """per kernel: average launch time and achieved algorithmic GB/s"""
     out = {}
     for k, (ms, n) in prof.items():
         if n == 0:
             continue
         avg = ms / n
-        b = algorithmic_bytes_per_cell(k, nv, mixed, nt)
+        b = algorithmic_bytes_per_cell(k, nv, mixed, nt, implicit_flag)
         gbs = b * cells / (avg * 1e-3) / 1e9 if b else None
         out[k] = {"ms_per_launch": avg, "launches": n, "bytes_per_cell": b, "GB_s": gbs,
                   "hbm_frac": gbs / HBM_PEAK_GBS if gbs else None}
@@ -640,7 +642,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     tot_ms, cnt = prof[dom]
     avg_s = tot_ms / cnt * 1e-3
-    own = algorithmic_bytes_per_cell(dom, r["nv"], form == "mixed")
+    own = algorithmic_bytes_per_cell(dom, r["nv"], form == "mixed", implicit_flag=iflag)
     bpc = IMPLICIT_SURVEY_BYTES if dom == "implicit" else own
     achieved = (bpc * r["cells"] / avg_s / 1e9) if bpc else None
     own_gbs = (own * r["cells"] / avg_s / 1e9) if own else None
@@ -665,10 +667,10 @@ def main():
         roof["fp64_valu"] = {"achieved": fl, "peak": 78.6, "unit": "TFLOP/s",
                              "frac": fl / 78.6,
                              "flop_per_cell": tp["implicit_fp64_flop_per_cell"]}
-    kernels = kernel_rooflines(prof, r["cells"], r["nv"], form == "mixed")
+    kernels = kernel_rooflines(prof, r["cells"], r["nv"], form == "mixed", implicit_flag=iflag)
     for k in kernels:
         kernels[k]["share"] = prof[k][0] / sum(x[0] for x in prof.values())
-    kernels_mx = kernel_rooflines(rmx["prof"], rmx["cells"], rmx["nv"], True)
+    kernels_mx = kernel_rooflines(rmx["prof"], rmx["cells"], rmx["nv"], True, implicit_flag=iflag)
     if not args.no_cpu:
         log("CPU baseline")
     cpu = None if args.no_cpu else cpu_baseline(workload_config, ex)
@@ -728,7 +730,8 @@ def main():
                        "fp64_vs_oracle": "bit-exact (tests/test_gpu_full_size.py)"},
         "fp64": {"value": value64, "ms_per_step": r64["sec"] * 1e3 / args.steps,
                  "dt_last_s": r64["dt"], "kernels": kernel_rooflines(r64["prof"], r64["cells"],
-                                                                   r64["nv"], False),
+                                                                   r64["nv"], False,
+                                                                   implicit_flag=iflag),
                  "tuned_launches": r64["tuned"]},
         "mixed": {"value": value_mx, "ms_per_step": rmx["sec"] * 1e3 / args.steps,
                   "dt_last_s": rmx["dt"], "kernels": kernels_mx,

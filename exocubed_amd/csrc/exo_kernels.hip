@@ -2509,9 +2509,10 @@ enum {
   K_PRIM_REST, K_PRIM_GHOST,
   // the partial (3x3) column correction
   K_IMPL3,
-  // within a stage, instances whose full column solve forms the stage's
-  // weighted average: their update (tendency only) and their solve; the
-  // update of every other instance
+  // within a stage: the ideal gas's update (tendency only; the column solve
+  // or u += du that follows forms the stage's weighted average), the full 5x5
+  // solve, the shallow-water update, the partial 3x3 solve and u += du without
+  // a solve (implicit_flag 0 or 8), the last three forming the average
   K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG,
   // within a stage, the FP64 ideal gas without scalars takes its x2 faces and
   // update in one kernel (k_face2_update); every other instance its separate
