@@ -563,6 +563,8 @@ def main():
     ap.add_argument("--only-headline", action="store_true")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="only the 256-instance shallow-water batch, both forms (for profiling)")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="only the serving stream at the default width, both forms")
     ap.add_argument("--large-steps", type=int, default=10)
     ap.add_argument("--implicit-flag", type=int, default=9, choices=(1, 9),
                     help="column correction of the headline workload: 9 full 5x5 (the "
@@ -606,6 +608,12 @@ def main():
                              "peak_active": r["peak_active"], "peak_bytes": r["peak_bytes"]})
                 print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
         print(json.dumps({"serving_sweep": rows, "requests": args.requests}))
+        return
+    if args.only_serving:
+        v64, _ = serving(ex, ex.PREC_FP64, args.requests, keep=0)
+        vmx, _ = serving(ex, ex.PREC_MIXED, args.requests, keep=0)
+        print(json.dumps({"serving": {"fp64": v64, "mixed": vmx,
+                                      "mixed_over_fp64": vmx["value"] / v64["value"]}}))
         return
     if args.only_ensemble:
         e64, _ = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)

@@ -813,16 +813,20 @@ __device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u
 // NOSCAL: the instance has no passive scalars (the caller checked nt == 0),
 // so the cell's update stores nothing and a caller can form several cells'
 // updates with shared loads.
-// X2R (FP64 only): the cell's two x2 face fluxes come from the caller's
-// registers (x2l: face j, x2r: face j + 1) instead of the flux array, and
-// its two x1 face fluxes too (x1l: face i, x1r: face i + 1).
+// Face fluxes a caller holds in registers (FP64): the cell's x2 faces j and
+// j + 1 (x2l, x2r), its x1 faces i and i + 1 (x1l, x1r), and the scalars' x2
+// faces (s2l, s2r; up to 4 scalars).
+struct RegFlux {
+  const double *x2l, *x2r, *x1l, *x1r, *s2l, *s2r;
+};
+constexpr int RF_NT = 4;  // scalars register fluxes carry (the configuration's maximum)
+
+// X2R (FP64 only): the fluxes of rf come from the caller's registers instead
+// of the flux arrays.
 template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out,
-                                            const double *x2l = nullptr,
-                                            const double *x2r = nullptr,
-                                            const double *x1l = nullptr,
-                                            const double *x1r = nullptr) {
+                                            const RegFlux *rf = nullptr) {
   double dt = st.dtv[0];
   double wght = sw.beta * dt;
   bool tend = g.eos == EOS_IDEAL;
@@ -866,10 +870,10 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     if (!MIX) {
       if (g.eos == EOS_SW) dflx = 0.;
       else
-        dflx = a1r * (X2R ? x1r[n] : st.flx[0][g.f(n, p, k, i + 1, j)]) -
-               a1l * (X2R ? x1l[n] : st.flx[0][g.f(n, p, k, i, j)]);
-      double f2r = X2R ? x2r[n] : st.flx[1][g.f(n, p, k, i, j + 1)];
-      double f2l = X2R ? x2l[n] : st.flx[1][g.f(n, p, k, i, j)];
+        dflx = a1r * (X2R ? rf->x1r[n] : st.flx[0][g.f(n, p, k, i + 1, j)]) -
+               a1l * (X2R ? rf->x1l[n] : st.flx[0][g.f(n, p, k, i, j)]);
+      double f2r = X2R ? rf->x2r[n] : st.flx[1][g.f(n, p, k, i, j + 1)];
+      double f2l = X2R ? rf->x2l[n] : st.flx[1][g.f(n, p, k, i, j)];
       dflx += a2r * f2r - a2l * f2l;
       dflx += a3r * st.flx[2][g.f(n, p, k + 1, i, j)] - a3l * st.flx[2][g.f(n, p, k, i, j)];
     } else {
@@ -955,8 +959,12 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       out[IVZ] += wght * a3c;
     }
   }
-  // passive scalars: weighted average and flux divergence into s
-  for (int n = 0; n < (NOSCAL ? 0 : g.nt); ++n) {
+  // passive scalars: weighted average and flux divergence into s (with
+  // register fluxes the loop runs over the largest count, so the flux arrays
+  // are indexed by constants and stay in registers)
+#pragma unroll
+  for (int n = 0; n < (NOSCAL ? 0 : X2R ? RF_NT : g.nt); ++n) {
+    if (X2R && n >= g.nt) break;
     size_t c = g.c(n, p, k, i, j);
     double s = st.s[c], s1;
     if (sw.first) {
@@ -976,7 +984,9 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     if (!MIX) {
       dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
       if (g.eos == EOS_SW) dflx = 0.;
-      dflx += a2r * st.sflx[1][g.f(n, p, k, i, j + 1)] - a2l * st.sflx[1][g.f(n, p, k, i, j)];
+      double s2r = X2R ? rf->s2r[n] : st.sflx[1][g.f(n, p, k, i, j + 1)];
+      double s2l = X2R ? rf->s2l[n] : st.sflx[1][g.f(n, p, k, i, j)];
+      dflx += a2r * s2r - a2l * s2l;
       dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
     } else {
       const ADDR(1) float *S0 = st.fsflx[0], *S1 = st.fsflx[1], *S2 = st.fsflx[2];
@@ -1071,18 +1081,21 @@ __global__ void __launch_bounds__(BLK) k_update_na(const Inst *__restrict__ I_, 
   }
 }
 
-// The x2 faces and the stage update (tendency only) of an FP64 ideal gas
-// without passive scalars in one pass, so the x2 face fluxes never go through
-// memory.  The pass writes du alone, which no face of the launch reads (a
-// shallow-water update here would race: it writes the primitive state the
-// neighbouring wavefronts' face stencils read).  The faces of the instance are numbered row by row (rows (p, k, i)
-// of N + 1 faces, j fastest); wavefront w takes faces 63 w .. 63 w + 63, lane
-// l face 63 w + l.  Lanes 0..62 then update the cell right of their face (its
-// face j), with face j + 1 from the next lane; a row's last face updates no
-// cell.  Every face and every cell is covered once, with the arithmetic of
-// face_flux64<1> and update_cell.
+// The x2 faces and the stage update (tendency only) of an FP64 ideal gas in
+// one pass, so the x2 face fluxes never go through memory; SCAL: with its
+// passive scalars (their x2 face fluxes too; the update averages s in place).
+// The pass writes du, s and s1, which no face of the launch reads (the faces
+// read w and r; a shallow-water update here would race: it writes the
+// primitive state the neighbouring wavefronts' face stencils read).  The
+// faces of the instance are numbered row by row (rows (p, k, i) of N + 1
+// faces, j fastest); wavefront w takes faces 63 w .. 63 w + 63, lane l face
+// 63 w + l.  Lanes 0..62 then update the cell right of their face (its face
+// j), with face j + 1 from the next lane; a row's last face updates no cell.
+// Every face and every cell is covered once, with the arithmetic of
+// face_flux64<1>, flux_h_body<1>'s scalar fluxes and update_cell.
 constexpr int F2_FACES = 63;  // cells a wavefront updates (faces per wavefront - 1)
 
+template <bool SCAL>
 __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &st, int lb,
                                                   StageW sw) {
   if (lb == 0 && threadIdx.x == 0) *st.bad = 0;  // as update_na_body
@@ -1112,22 +1125,43 @@ __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &
   }
   double fl[5] = {0., 0., 0., 0., 0.}, fr[5];
   if (act) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
+  // scalar face fluxes: WENO5 of r across the face, upwinded by the mass flux
+  double sl[RF_NT] = {0., 0., 0., 0.}, sr[RF_NT] = {0., 0., 0., 0.};
+  if (SCAL && act) {
+#pragma unroll
+    for (int n = 0; n < RF_NT; ++n) {
+      if (n >= g.nt) break;
+      double q[6];
+#pragma unroll
+      for (int s = 0; s < 6; ++s) q[s] = st.r[g.c(n, p, k, i, j - 3 + s)];
+      double rl = weno5(q[4], q[3], q[2], q[1], q[0]);
+      double rr = weno5(q[1], q[2], q[3], q[4], q[5]);
+      double m = fl[IDN];
+      sl[n] = m >= 0.0 ? m * rl : m * rr;
+    }
+  }
   // every lane takes part in the exchange
 #pragma unroll
   for (int n = 0; n < 5; ++n) fr[n] = __shfl_down(fl[n], 1);
+  if (SCAL) {
+#pragma unroll
+    for (int n = 0; n < RF_NT; ++n) sr[n] = __shfl_down(sl[n], 1);
+  }
   if (cell) {
     double out[5];
-    update_cell<false, true, true, true>(g, st, p, k, i, j, sw, out, fl, fr, f1l, f1r);
+    RegFlux rf{fl, fr, f1l, f1r, sl, sr};
+    update_cell<false, true, !SCAL, true>(g, st, p, k, i, j, sw, out, &rf);
 #pragma unroll
     for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i, j)] = out[n];
   }
 }
 
+template <bool SCAL>
 __global__ void __launch_bounds__(BLK) k_face2_update(const Inst *__restrict__ I_, Work w,
                                                       StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    face2_update_body(g, st, lb, sw);
+    face2_update_body<SCAL>(g, st, lb, sw);
   }
 }
 
@@ -2514,10 +2548,10 @@ enum {
   // solve, the shallow-water update, the partial 3x3 solve and u += du without
   // a solve (implicit_flag 0 or 8), the last three forming the average
   K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG,
-  // within a stage, the FP64 ideal gas without scalars takes its x2 faces and
-  // update in one kernel (k_face2_update); every other instance its separate
-  // x2 face kernel and update
-  K_FUSE2, K_FACE_NF, K_UPD_NA_NF, K_NKIND
+  // within a stage, the FP64 ideal gas takes its x2 faces and update in one
+  // kernel (k_face2_update, without and with scalars), FP64 shallow water its
+  // separate x2 face kernel (the mixed forms have their own kinds)
+  K_FUSE2, K_FACE_NF, K_FUSE2_S, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2656,16 +2690,13 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, int shape
     case K_IMPL3_AVG:
       return ideal && (g.implicit & 1) && !(g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
     case K_ADD_DU_AVG: return ideal && !(g.implicit & 1) ? (int)grid1(cells) : 0;
-    case K_FUSE2: {
-      if (!ideal || mixed || g.nt) return 0;
+    case K_FUSE2:
+    case K_FUSE2_S: {
+      if (!ideal || mixed || (kind == K_FUSE2) != (g.nt == 0)) return 0;
       size_t waves = (faces + F2_FACES - 1) / F2_FACES;
       return (int)((waves + BLK / 64 - 1) / (BLK / 64));
     }
-    case K_FACE_NF: return !mixed && !(ideal && !g.nt) ? (int)grid1(faces) : 0;
-    case K_UPD_NA_NF: {
-      int lv = shape ? shape : UPD_NA_L;
-      return ideal && !mixed && g.nt ? (int)grid1(cells / g.nx1 * ((g.nx1 + lv - 1) / lv)) : 0;
-    }
+    case K_FACE_NF: return !mixed && !ideal ? (int)grid1(faces) : 0;
     // implicit_flag bit 1 solves the x1 columns, bit 8 with the full 5x5
     // system (solve_implicit_3d.cpp:102-135); without bit 1, u += du
     case K_IMPL: return ideal && (g.implicit & 1) && (g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
@@ -2741,8 +2772,7 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
 // a kind's block shape; the block table of a kind whose block count depends
 // on it (the stage updates) is rebuilt on the device
 void set_shape(Launcher &L, int kind, int shape) {
-  bool x1 = kind == K_X1 || kind == K_X1_MIX,
-       upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA || kind == K_UPD_NA_NF;
+  bool x1 = kind == K_X1 || kind == K_X1_MIX, upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
   bool ok = shape == 0 || (x1 && (shape == 2 || shape == 4 || shape == 8)) ||
             (upd && (shape == 1 || shape == 2 || shape == 4));
   if (!ok) throw std::string("no block shape ") + std::to_string(shape) + " for launch kind " +
@@ -2992,7 +3022,7 @@ void launch_update(Launcher &L, int stage, int prim) {
 
 // the ideal gas's stage update (tendency only) with its tuned levels per thread
 void launch_update_na(Launcher &L, int kind, int kid, StageW sw) {
-  bool mix = kind == K_UPD_MIX_NA;  // K_UPD_NA and K_UPD_NA_NF are FP64
+  bool mix = kind == K_UPD_MIX_NA;
   switch (L.shape[kind] ? L.shape[kind] : UPD_NA_L) {
     case 2:
       if (mix) launch(L, kind, kid, k_update_na<true, 2>, BLK, 0, sw);
@@ -3017,8 +3047,8 @@ void launch_update_implicit(Launcher &L, int stage, bool fused = false) {
             1, DFLOOR, SFLOOR};
   launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
   if (fused) {
-    launch(L, K_FUSE2, KID_UPDATE, k_face2_update, BLK, 0, sw);
-    launch_update_na(L, K_UPD_NA_NF, KID_UPDATE, sw);
+    launch(L, K_FUSE2, KID_UPDATE, k_face2_update<false>, BLK, 0, sw);
+    launch(L, K_FUSE2_S, KID_UPDATE, k_face2_update<true>, BLK, 0, sw);
   } else {
     launch_update_na(L, K_UPD_NA, KID_UPDATE, sw);
   }
@@ -3133,7 +3163,7 @@ void tune_launcher(Launcher &L, int reps) {
             case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
             case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
             case K_UPD_NA: case K_UPD_MIX_NA: launch_update_na(L, kind, -1, sw); break;
-            case K_FUSE2: launch(L, K_FUSE2, -1, k_face2_update, BLK, 0, sw); break;
+            case K_FUSE2: launch(L, K_FUSE2, -1, k_face2_update<false>, BLK, 0, sw); break;
           }
         }
         HIPCHK(hipEventRecord(e1, L.stream));
