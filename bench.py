@@ -64,12 +64,13 @@ def algorithmic_bytes_per_cell(kernel, nv, mixed=False, nt=0, implicit_flag=9):
         # read w (FP64 form) or fw (mixed), write the face flux
         "flux_x2": nv * f + nv * f + nt * (f + f),
         "flux_x3": nv * f + nv * f + nt * (f + f),
-        # within a stage with the full column solve: read w, F1, F2, F3 (+ psf),
-        # write du; scalars s, s1, G1..3 (the weighted average of u is formed
-        # in the column solve).  The FP64 ideal gas without scalars forms its
-        # x2 faces in the same kernel (k_face2_update): read w, F1, F3, write du
-        "update": (4 * nv * 8 if not mixed and nt == 0 else
-                   2 * nv * 8 + 3 * nv * f + (8 if mixed else 0) + nt * (4 * 8 + 3 * f)),
+        # within a stage the ideal gas forms its x2 faces and tendency in one
+        # kernel (k_face2_update; the weighted average of u is formed in the
+        # column step): read w, F1, F3 (mixed: also the FP32 copy fw the faces
+        # read, and the FP64 x1 face pressure), write du; per scalar r (or its
+        # FP32 copy), s, s1, G1, G3
+        "update": (4 * nv * 8 + nt * (5 * 8) if not mixed else
+                   2 * nv * 8 + nv * 4 + 2 * nv * f + 8 + nt * (4 + 2 * 8 + 2 * f)),
         # the column step of the stage (KID_IMPLICIT): k_implicit<true> (5x5,
         # implicit_flag 9) and k_implicit3<true> (3x3, implicit_flag 1) read du,
         # w, u, u1 (stages 2-3) and write u, or read du, w, u and write u, u1
@@ -97,18 +98,17 @@ def step_compulsory_bytes_per_cell(nv, mixed):
 
 # kernel of the live profile -> its name in the rocprofv3 PMC summary, per form
 PMC_NAMES = {
-    "fp64": {"implicit": "k_implicit<true>", "update": "k_face2_update",
+    "fp64": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, false>",
              "flux_x1": "k_flux_x1<4>", "flux_x3": "k_flux_h<2>",
              "edge_states": "k_edge_states", "ghost": "k_ghost",
              "prim": "k_prim", "dt": "k_dt"},
-    "mixed": {"implicit": "k_implicit<true>", "update": "k_update_na<true, 1>",
-              "flux_x1": "k_flux_x1_mix<4>", "flux_x2": "k_flux_h_mix<1, false>",
-              "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
+    "mixed": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, true>",
+              "flux_x1": "k_flux_x1_mix<4>", "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
               "dt": "k_dt"},
 }
 # launches of each kernel per SSPRK3 step of the headline workload (a kernel
-# a form does not launch is absent from its PMC_NAMES: the FP64 x2 faces are
-# formed within its update)
+# a form does not launch is absent from its PMC_NAMES: the x2 faces are formed
+# within the update)
 STEP_LAUNCHES = {"implicit": 3, "update": 3, "flux_x1": 3, "flux_x2": 3, "flux_x3": 3,
                  "edge_states": 3, "ghost": 3, "prim": 3, "dt": 1}
 TRAFFIC_PATH = os.path.join(ROOT, "profiles", "traffic_exo3_96x96x64_lmars_implicit.json")

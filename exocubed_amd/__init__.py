@@ -26,8 +26,9 @@ KERNELS = ("prim", "ghost", "flux_x1", "edge_states", "flux_x2", "flux_x3", "upd
 LAUNCH_KINDS = ("prim", "ghost_u", "ghost_s", "x1", "x1_mix", "edge", "edge_mix", "face",
                 "face_mix", "update", "update_mix", "implicit", "add_du", "cells", "one", "dt",
                 "edge_flux", "face_mix_sw", "prim_rest", "prim_ghost", "implicit3",
-                "update_na", "update_mix_na", "implicit_avg", "update_rs", "implicit3_avg",
-                "add_du_avg", "face2_update", "face_x2", "face2_update_s")
+                "implicit_avg", "update_rs", "implicit3_avg",
+                "add_du_avg", "face2_update", "face_x2", "face2_update_s",
+                "face2_update_mix", "face2_update_mix_s")
 
 
 def _tuning(caps):

@@ -818,11 +818,13 @@ __device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u
 // faces (s2l, s2r; up to 4 scalars).
 struct RegFlux {
   const double *x2l, *x2r, *x1l, *x1r, *s2l, *s2r;
+  const float *f2l, *f2r, *fs2l, *fs2r;  // the mixed form's FP32 x2 face fluxes
 };
 constexpr int RF_NT = 4;  // scalars register fluxes carry (the configuration's maximum)
 
-// X2R (FP64 only): the fluxes of rf come from the caller's registers instead
-// of the flux arrays.
+// X2R: the fluxes of rf come from the caller's registers instead of the flux
+// arrays (FP64: x2, x1 and the scalars' x2; mixed: the FP32 x2 and scalar x2
+// faces).
 template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out,
@@ -882,7 +884,8 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       // FP64 update)
       const ADDR(1) float *F0 = st.fflx[0], *F1 = st.fflx[1], *F2 = st.fflx[2];
       dflx = a1r * (double)F0[g.f(n, p, k, i + 1, j)] - a1l * (double)F0[g.f(n, p, k, i, j)];
-      dflx += a2r * (double)F1[g.f(n, p, k, i, j + 1)] - a2l * (double)F1[g.f(n, p, k, i, j)];
+      dflx += a2r * (double)(X2R ? rf->f2r[n] : F1[g.f(n, p, k, i, j + 1)]) -
+              a2l * (double)(X2R ? rf->f2l[n] : F1[g.f(n, p, k, i, j)]);
       dflx += a3r * (double)F2[g.f(n, p, k + 1, i, j)] - a3l * (double)F2[g.f(n, p, k, i, j)];
       double P = st.lref[i];
       if (n == IVX)
@@ -993,7 +996,8 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       dflx = g.eos == EOS_SW ? 0.
                              : a1r * (double)S0[g.f(n, p, k, i + 1, j)] -
                                    a1l * (double)S0[g.f(n, p, k, i, j)];
-      dflx += a2r * (double)S1[g.f(n, p, k, i, j + 1)] - a2l * (double)S1[g.f(n, p, k, i, j)];
+      dflx += a2r * (double)(X2R ? rf->fs2r[n] : S1[g.f(n, p, k, i, j + 1)]) -
+              a2l * (double)(X2R ? rf->fs2l[n] : S1[g.f(n, p, k, i, j)]);
       dflx += a3r * (double)S2[g.f(n, p, k + 1, i, j)] - a3l * (double)S2[g.f(n, p, k, i, j)];
     }
     s -= div_rn(wght * dflx, vol, yvol);
@@ -1034,51 +1038,61 @@ __global__ void __launch_bounds__(BLK) k_update(const Inst *__restrict__ I_, Wor
   }
 }
 
-// The ideal gas's update within a stage (the tendency only, as
-// update_body<MIX, true>), LV consecutive levels of one column per thread
-// (UPD_NA_L by default, 1, 2 or 4 as the autotuner picks).  With more levels
-// per thread the levels' tendencies are all formed before any is stored, so
-// loads the levels share are issued once; measured on 6 x 96 x 96 x 64 one
-// level per thread is fastest (more cells in flight per SIMD).
-constexpr int UPD_NA_L = 1;
-
-template <bool MIX, int LV>
-__device__ __forceinline__ void update_na_body(const DGrid &g, const DState &st, int lb,
-                                               StageW sw) {
-  size_t t = (size_t)lb * BLK + threadIdx.x;
-  if (t == 0) *st.bad = 0;  // as update_body resets it
-  int N = g.N, nch = (g.nx1 + LV - 1) / LV;
-  size_t total = (size_t)6 * N * nch * N;
-  if (t >= total) return;
-  int j = g.js + (int)(t % N);
-  size_t r = t / N;
-  int i0 = g.is + (int)(r % nch) * LV;
-  r /= nch;
-  int k = g.ks + (int)(r % N), p = (int)(r / N);
-  double out[LV][5];
-  if (g.nt == 0) {
+// FP32 flux of the mixed ideal gas at the x2 (dir 1) or x3 (dir 2) face (p,
+// k, i, j), j = js + fa (x2) or k = ks + fa (x3), along-edge index kk: WENO5
+// of the FP32 perturbation copy fw, the neighbour panel's exchanged FP32
+// state on a panel edge, the level's reference density added back and its
+// reference pressure P split off (riemann_pert, exo_mixed.hpp).
+template <int dir>
+__device__ __forceinline__ void face_flux_mix(const DGrid &g, const DState &st, int p, int kk,
+                                              int fa, int i, int k, int j, float *fl) {
+#pragma clang fp contract(fast)
+  int N = g.N, ni = g.nx1;
+  float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-    for (int q = 0; q < LV; ++q)
-      if (i0 + q <= g.ie) update_cell<MIX, true, true>(g, st, p, k, i0 + q, j, sw, out[q]);
+  for (int n = 0; n < 5; ++n) {
+    if (g.skipv(n)) continue;
+    float v[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      int ks = dir == 1 ? k : k - 3 + s, js = dir == 1 ? j - 3 + s : j;
+      v[s] = st.fw[g.c(n, p, ks, i, js)];
+    }
+    wl[n] = weno5f(v[4], v[3], v[2], v[1], v[0]);
+    wr[n] = weno5f(v[1], v[2], v[3], v[4], v[5]);
+  }
+  size_t vs = (size_t)6 * 4 * N * ni;
+  int lo_side = dir == 1 ? 0 : 2;
+  // panel edges: the neighbour's state
+  if (fa == 0) {
+    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (!g.skipv(n)) wl[n] = st.fedge[n * vs + b];
+  } else if (fa == N) {
+    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (!g.skipv(n)) wr[n] = st.fedge[n * vs + b];
+  }
+  // full density
+  float rref = (float)st.lref[2 * g.nc1 + i];
+  wl[IDN] += rref;
+  wr[IDN] += rref;
+  float P = (float)st.lref[g.nc1 + i];
+  double cosd, sind, cthd;
+  if (dir == 1) {
+    cosd = g.T(T_COS2, p, k, j);
+    sind = g.T(T_SIN2, p, k, j);
+    cthd = g.T(T_CTH2, p, k, j);
   } else {
-#pragma unroll
-    for (int q = 0; q < LV; ++q)
-      if (i0 + q <= g.ie) update_cell<MIX, true>(g, st, p, k, i0 + q, j, sw, out[q]);
+    cosd = g.T(T_COS3, p, k, j);
+    sind = g.T(T_SIN3, p, k, j);
+    cthd = g.T(T_CTH3, p, k, j);
   }
-#pragma unroll
-  for (int q = 0; q < LV; ++q)
-    if (i0 + q <= g.ie)
-#pragma unroll
-      for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i0 + q, j)] = out[q][n];
-}
-
-template <bool MIX, int LV>
-__global__ void __launch_bounds__(BLK) k_update_na(const Inst *__restrict__ I_, Work w,
-                                                   StageW sw) {
-  WORK_LOOP(w) {
-    WORK_INST(w);
-    update_na_body<MIX, LV>(g, st, lb, sw);
-  }
+  float cosf = (float)cosd, sinf = (float)sind, cth = (float)cthd;
+  riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf, sinf,
+               0.f);
 }
 
 // The x2 faces and the stage update (tendency only) of an FP64 ideal gas in
@@ -1095,10 +1109,10 @@ __global__ void __launch_bounds__(BLK) k_update_na(const Inst *__restrict__ I_, 
 // face_flux64<1>, flux_h_body<1>'s scalar fluxes and update_cell.
 constexpr int F2_FACES = 63;  // cells a wavefront updates (faces per wavefront - 1)
 
-template <bool SCAL>
+template <bool SCAL, bool MIX>
 __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &st, int lb,
                                                   StageW sw) {
-  if (lb == 0 && threadIdx.x == 0) *st.bad = 0;  // as update_na_body
+  if (lb == 0 && threadIdx.x == 0) *st.bad = 0;  // as update_body
   int N = g.N, ni = g.nx1, na = N + 1;
   size_t total = (size_t)6 * N * ni * na;
   int lane = threadIdx.x & 63;
@@ -1112,56 +1126,82 @@ __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &
   int kk = (int)(r % N), p = (int)(r / N);
   int k = g.ks + kk, j = g.js + fa;
   bool cell = act && lane < F2_FACES && fa < N;
-  // the update's x1 face fluxes, loaded before the face arithmetic so their
-  // latency is spent on it (the x3 ones too would take the kernel from four to
-  // three wavefronts per SIMD, which measured slower)
+  // the update's x1 face fluxes (FP64 form), loaded before the face
+  // arithmetic so their latency is spent on it (the x3 ones too would take the
+  // kernel from four to three wavefronts per SIMD, which measured slower)
   double f1l[5] = {0., 0., 0., 0., 0.}, f1r[5] = {0., 0., 0., 0., 0.};
-  if (cell) {
+  if (!MIX && cell) {
 #pragma unroll
     for (int n = 0; n < 5; ++n) {
       f1l[n] = st.flx[0][g.f(n, p, k, i, j)];
       f1r[n] = st.flx[0][g.f(n, p, k, i + 1, j)];
     }
   }
-  double fl[5] = {0., 0., 0., 0., 0.}, fr[5];
-  if (act) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
-  // scalar face fluxes: WENO5 of r across the face, upwinded by the mass flux
+  // the face flux (FP64, or the mixed form's FP32) and the scalars' (WENO5 of
+  // r, or of its FP32 copy, across the face, upwinded by the mass flux); every
+  // lane takes part in the exchanges
+  double fl[5] = {0., 0., 0., 0., 0.}, fr[5] = {0., 0., 0., 0., 0.};
   double sl[RF_NT] = {0., 0., 0., 0.}, sr[RF_NT] = {0., 0., 0., 0.};
-  if (SCAL && act) {
+  float gl[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, gr[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float tl[RF_NT] = {0.f, 0.f, 0.f, 0.f}, tr[RF_NT] = {0.f, 0.f, 0.f, 0.f};
+  if (MIX) {
+    if (act) face_flux_mix<1>(g, st, p, kk, fa, i, k, j, gl);
+    if (SCAL && act) {
 #pragma unroll
-    for (int n = 0; n < RF_NT; ++n) {
-      if (n >= g.nt) break;
-      double q[6];
+      for (int n = 0; n < RF_NT; ++n) {
+        if (n >= g.nt) break;
+        float q[6];
 #pragma unroll
-      for (int s = 0; s < 6; ++s) q[s] = st.r[g.c(n, p, k, i, j - 3 + s)];
-      double rl = weno5(q[4], q[3], q[2], q[1], q[0]);
-      double rr = weno5(q[1], q[2], q[3], q[4], q[5]);
-      double m = fl[IDN];
-      sl[n] = m >= 0.0 ? m * rl : m * rr;
+        for (int s = 0; s < 6; ++s) q[s] = st.fw[g.c(g.nv + n, p, k, i, j - 3 + s)];
+        float rl = weno5f(q[4], q[3], q[2], q[1], q[0]);
+        float rr = weno5f(q[1], q[2], q[3], q[4], q[5]);
+        float m = gl[IDN];
+        tl[n] = m >= 0.f ? m * rl : m * rr;
+      }
     }
-  }
-  // every lane takes part in the exchange
 #pragma unroll
-  for (int n = 0; n < 5; ++n) fr[n] = __shfl_down(fl[n], 1);
-  if (SCAL) {
+    for (int n = 0; n < 5; ++n) gr[n] = __shfl_down(gl[n], 1);
+    if (SCAL) {
 #pragma unroll
-    for (int n = 0; n < RF_NT; ++n) sr[n] = __shfl_down(sl[n], 1);
+      for (int n = 0; n < RF_NT; ++n) tr[n] = __shfl_down(tl[n], 1);
+    }
+  } else {
+    if (act) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
+    if (SCAL && act) {
+#pragma unroll
+      for (int n = 0; n < RF_NT; ++n) {
+        if (n >= g.nt) break;
+        double q[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) q[s] = st.r[g.c(n, p, k, i, j - 3 + s)];
+        double rl = weno5(q[4], q[3], q[2], q[1], q[0]);
+        double rr = weno5(q[1], q[2], q[3], q[4], q[5]);
+        double m = fl[IDN];
+        sl[n] = m >= 0.0 ? m * rl : m * rr;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 5; ++n) fr[n] = __shfl_down(fl[n], 1);
+    if (SCAL) {
+#pragma unroll
+      for (int n = 0; n < RF_NT; ++n) sr[n] = __shfl_down(sl[n], 1);
+    }
   }
   if (cell) {
     double out[5];
-    RegFlux rf{fl, fr, f1l, f1r, sl, sr};
-    update_cell<false, true, !SCAL, true>(g, st, p, k, i, j, sw, out, &rf);
+    RegFlux rf{fl, fr, f1l, f1r, sl, sr, gl, gr, tl, tr};
+    update_cell<MIX, true, !SCAL, true>(g, st, p, k, i, j, sw, out, &rf);
 #pragma unroll
     for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i, j)] = out[n];
   }
 }
 
-template <bool SCAL>
+template <bool SCAL, bool MIX>
 __global__ void __launch_bounds__(BLK) k_face2_update(const Inst *__restrict__ I_, Work w,
                                                       StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    face2_update_body<SCAL>(g, st, lb, sw);
+    face2_update_body<SCAL, MIX>(g, st, lb, sw);
   }
 }
 
@@ -1281,7 +1321,9 @@ struct ColIOShim {
 // lane's base of the coefficient scratch, [level][pair][64 lanes]: the 30
 // doubles of a level (a^-1 c row-major, then delta) as 15 pairs, each pair 64
 // lanes x 16 B contiguous, so a level is 15 coalesced 16-byte stores, and 15
-// loads back.
+// loads back.  Both are non-temporal: each coefficient is written once and read
+// once a whole sweep later, and streaming it past the caches leaves them to
+// the state (the solve 0.61 -> 0.56 ms on 6 x 96 x 96 x 64).
 template <bool AVG, typename IO>
 __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double dt, double gamma,
                                            double grav, ADDR(1) dpair *__restrict__ CFP) {
@@ -1413,7 +1455,8 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
 #pragma unroll
       for (int n = 0; n < 5; ++n) cv[25 + n] = delta.a[n];
 #pragma unroll
-      for (int q2 = 0; q2 < 15; ++q2) *cfp(i, q2) = dpair{cv[2 * q2], cv[2 * q2 + 1]};
+      for (int q2 = 0; q2 < 15; ++q2)
+        __builtin_nontemporal_store(dpair{cv[2 * q2], cv[2 * q2 + 1]}, cfp(i, q2));
     }
     Am = Ap;
     em = e0;
@@ -1449,7 +1492,7 @@ __device__ __forceinline__ void col5_solve(const IO &io, int is, int ie, double 
     double cv[30];
 #pragma unroll
     for (int q2 = 0; q2 < 15; ++q2) {
-      dpair t = *cfp(l, q2);
+      dpair t = __builtin_nontemporal_load(cfp(l, q2));
       cv[2 * q2] = t.x;
       cv[2 * q2 + 1] = t.y;
     }
@@ -1702,7 +1745,8 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
 #pragma unroll
     for (int n = 0; n < 3; ++n) cv[9 + n] = delta.a[n];
 #pragma unroll
-    for (int q2 = 0; q2 < 6; ++q2) *cfp(i, q2) = dpair{cv[2 * q2], cv[2 * q2 + 1]};
+    for (int q2 = 0; q2 < 6; ++q2)
+      __builtin_nontemporal_store(dpair{cv[2 * q2], cv[2 * q2 + 1]}, cfp(i, q2));
     Am = Ap;
     em = e0;
     e0 = ep;
@@ -1748,7 +1792,7 @@ __device__ __forceinline__ void implicit3_body(const DGrid &g, const DState &st,
   auto bload = [&](int l, double *cv, double *uo) {
 #pragma unroll
     for (int q2 = 0; q2 < 6; ++q2) {
-      dpair t = *cfp(l, q2);
+      dpair t = __builtin_nontemporal_load(cfp(l, q2));
       cv[2 * q2] = t.x;
       cv[2 * q2 + 1] = t.y;
     }
@@ -2307,6 +2351,25 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
   }
   int k = dir == 1 ? g.ks + kk : g.ks + fa;
   int j = dir == 1 ? g.js + fa : g.js + kk;
+  if (!SW) {
+    float fl[5];
+    face_flux_mix<dir>(g, st, p, kk, fa, i, k, j, fl);
+    float m = fl[IDN];
+    ADDR(1) float *F = st.fflx[dir];
+#pragma unroll
+    for (int n = 0; n < 5; ++n)
+      if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
+    for (int n = 0; n < g.nt; ++n) {
+      float r[6];
+      int nr = g.nv + n;  // the tracer's FP32 copy
+      for (int s = 0; s < 6; ++s)
+        r[s] = dir == 1 ? st.fw[g.c(nr, p, k, i, j - 3 + s)] : st.fw[g.c(nr, p, k - 3 + s, i, j)];
+      float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
+      float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
+      st.fsflx[dir][g.f(n, p, k, i, j)] = m >= 0.f ? m * rl : m * rr;
+    }
+    return;
+  }
   float wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0}, fl[5];
   // shallow water: deltas about the face's left cell (riemann_sw_delta)
   constexpr bool sw = SW;
@@ -2385,22 +2448,6 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
       st.sflx[dir][g.f(n, p, k, i, j)] = m >= 0.0 ? m * (double)rl : m * (double)rr;
     }
     return;
-  }
-  riemann_pert(g.eos, g.rs, dir == 1 ? IVY : IVZ, (float)g.gamma, wl, wr, P, fl, cth, cosf, sinf,
-               0.f);
-  float m = fl[IDN];
-  ADDR(1) float *F = st.fflx[dir];
-#pragma unroll
-  for (int n = 0; n < 5; ++n)
-    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
-  for (int n = 0; n < g.nt; ++n) {
-    float r[6];
-    int nr = g.nv + n;  // the tracer's FP32 copy
-    for (int s = 0; s < 6; ++s)
-      r[s] = dir == 1 ? st.fw[g.c(nr, p, k, i, j - 3 + s)] : st.fw[g.c(nr, p, k - 3 + s, i, j)];
-    float rl = weno5f(r[4], r[3], r[2], r[1], r[0]);
-    float rr = weno5f(r[1], r[2], r[3], r[4], r[5]);
-    st.fsflx[dir][g.f(n, p, k, i, j)] = m >= 0.f ? m * rl : m * rr;
   }
 }
 
@@ -2543,15 +2590,15 @@ enum {
   K_PRIM_REST, K_PRIM_GHOST,
   // the partial (3x3) column correction
   K_IMPL3,
-  // within a stage: the ideal gas's update (tendency only; the column solve
-  // or u += du that follows forms the stage's weighted average), the full 5x5
-  // solve, the shallow-water update, the partial 3x3 solve and u += du without
-  // a solve (implicit_flag 0 or 8), the last three forming the average
-  K_UPD_NA, K_UPD_MIX_NA, K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG,
+  // within a stage: the full 5x5 solve, the shallow-water update, the partial
+  // 3x3 solve and u += du without a solve (implicit_flag 0 or 8); the column
+  // steps form the stage's weighted average of u (the ideal gas's update,
+  // k_face2_update below, forms only the tendency)
+  K_IMPL_AVG, K_UPD_RS, K_IMPL3_AVG, K_ADD_DU_AVG,
   // within a stage, the FP64 ideal gas takes its x2 faces and update in one
   // kernel (k_face2_update, without and with scalars), FP64 shallow water its
   // separate x2 face kernel (the mixed forms have their own kinds)
-  K_FUSE2, K_FACE_NF, K_FUSE2_S, K_NKIND
+  K_FUSE2, K_FACE_NF, K_FUSE2_S, K_FUSE2_MIX, K_FUSE2_MIX_S, K_NKIND
 };
 
 // kernel ids of the profile (exo_profile_end)
@@ -2571,11 +2618,9 @@ struct Launcher {
   int *d_bstart = nullptr;  // [K_NKIND][ninst + 1]
   int nblk[K_NKIND] = {};
   int cap[K_NKIND] = {};    // workgroups of a persistent launch (0: one per block)
-  int shape[K_NKIND] = {};  // tuned block shape of a kind (0: the default); the x1
-                            // kinds: waves per 64-column block (2, 4 or 8); the
-                            // stage updates: levels per thread (1, 2 or 4)
-  std::vector<Inst> h_inst;     // host copies of the instances and their forms,
-  std::vector<char> h_mixed;    // for block tables rebuilt when a shape changes
+  int shape[K_NKIND] = {};  // tuned block shape of a kind (0: the default): the x1
+                            // kinds' waves per 64-column block (2, 4 or 8)
+  std::vector<Inst> h_inst;     // host copies of the instances (the tuner reads them)
   size_t lds_x1 = 0;        // dynamic LDS of the x1 kernels (largest instance)
   // live per-kernel timing with HIP events on the stream
   bool prof = false;
@@ -2654,7 +2699,7 @@ hipEvent_t next_event(Launcher &L) {
 }
 
 // blocks of one instance in a launch of the given kind (0: not applicable)
-int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, int shape = 0) {
+int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned) {
   bool ideal = g.eos == EOS_IDEAL;
   size_t cells = (size_t)6 * g.N * g.N * g.nx1;
   size_t faces = (size_t)6 * g.N * g.nx1 * (g.N + 1);
@@ -2679,20 +2724,18 @@ int kind_blocks(int kind, const Grid &g, bool mixed, int ngd, int ned, int shape
     // mixed shallow water stores FP64 face fluxes: the FP64 update
     case K_UPD: return !mixed || !ideal ? (int)grid1(cells) : 0;
     case K_UPD_MIX: return mixed && ideal ? (int)grid1(cells) : 0;
-    case K_UPD_NA:
-    case K_UPD_MIX_NA: {
-      int lv = shape ? shape : UPD_NA_L;
-      bool on = ideal && (kind == K_UPD_MIX_NA) == mixed;
-      return on ? (int)grid1(cells / g.nx1 * ((g.nx1 + lv - 1) / lv)) : 0;
-    }
     case K_IMPL_AVG: return full ? (int)grid1(cols, 64) * 6 : 0;
     case K_UPD_RS: return !ideal ? (int)grid1(cells) : 0;
     case K_IMPL3_AVG:
       return ideal && (g.implicit & 1) && !(g.implicit & 8) ? (int)grid1(cols, 64) * 6 : 0;
     case K_ADD_DU_AVG: return ideal && !(g.implicit & 1) ? (int)grid1(cells) : 0;
     case K_FUSE2:
-    case K_FUSE2_S: {
-      if (!ideal || mixed || (kind == K_FUSE2) != (g.nt == 0)) return 0;
+    case K_FUSE2_S:
+    case K_FUSE2_MIX:
+    case K_FUSE2_MIX_S: {
+      bool mk = kind == K_FUSE2_MIX || kind == K_FUSE2_MIX_S;
+      bool sk = kind == K_FUSE2_S || kind == K_FUSE2_MIX_S;
+      if (!ideal || mixed != mk || sk != (g.nt > 0)) return 0;
       size_t waves = (faces + F2_FACES - 1) / F2_FACES;
       return (int)((waves + BLK / 64 - 1) / (BLK / 64));
     }
@@ -2730,13 +2773,11 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     L.lds_x1 = std::max(L.lds_x1, (size_t)(h->g.nc1 + 1) * 64 * sizeof(double));
   }
   L.h_inst = inst;
-  L.h_mixed.assign(n, 0);
-  for (int i = 0; i < n; ++i) L.h_mixed[i] = hs[i]->mixed;
   for (int kind = 0; kind < K_NKIND; ++kind) {
     long long acc = 0;
     for (int i = 0; i < n; ++i) {
       tab[(size_t)kind * (n + 1) + i] = (int)acc;
-      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned, L.shape[kind]);
+      acc += kind_blocks(kind, hs[i]->g, hs[i]->mixed, inst[i].ngd, inst[i].ned);
     }
     if (acc > 0x7fffffffLL) throw std::string("work list too large for one launch");
     tab[(size_t)kind * (n + 1) + n] = (int)acc;
@@ -2769,28 +2810,14 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
   HIPCHK(hipStreamSynchronize(L.stream));
 }
 
-// a kind's block shape; the block table of a kind whose block count depends
-// on it (the stage updates) is rebuilt on the device
+// a kind's block shape (the x1 sweeps' waves per block; the block count of a
+// launch does not depend on it)
 void set_shape(Launcher &L, int kind, int shape) {
-  bool x1 = kind == K_X1 || kind == K_X1_MIX, upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
-  bool ok = shape == 0 || (x1 && (shape == 2 || shape == 4 || shape == 8)) ||
-            (upd && (shape == 1 || shape == 2 || shape == 4));
+  bool x1 = kind == K_X1 || kind == K_X1_MIX;
+  bool ok = shape == 0 || (x1 && (shape == 2 || shape == 4 || shape == 8));
   if (!ok) throw std::string("no block shape ") + std::to_string(shape) + " for launch kind " +
                std::to_string(kind);
   L.shape[kind] = shape;
-  if (!upd) return;
-  int n = L.ninst;
-  std::vector<int> row(n + 1);
-  long long acc = 0;
-  for (int i = 0; i < n; ++i) {
-    row[i] = (int)acc;
-    acc += kind_blocks(kind, L.h_inst[i].g, L.h_mixed[i], L.h_inst[i].ngd, L.h_inst[i].ned, shape);
-  }
-  row[n] = (int)acc;
-  L.nblk[kind] = (int)acc;
-  HIPCHK(hipMemcpyAsync(L.d_bstart + (size_t)kind * (n + 1), row.data(), sizeof(int) * (n + 1),
-                        hipMemcpyHostToDevice, L.stream));
-  HIPCHK(hipStreamSynchronize(L.stream));
 }
 
 // one launch of a work-list kernel; kid < 0: not timed
@@ -2997,7 +3024,7 @@ void launch_fluxes(Launcher &L, bool fused = false) {
   launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
   launch(L, fused ? K_FACE_NF : K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
   launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
-  launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1, false>, BLK, 0);
+  if (!fused) launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1, false>, BLK, 0);
   launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2, false>, BLK, 0);
   launch(L, K_FACE_MIX_SW, KID_FLUX_X2, k_flux_h_mix<1, true>, BLK, 0);
   launch(L, K_FACE_MIX_SW, KID_FLUX_X3, k_flux_h_mix<2, true>, BLK, 0);
@@ -3020,39 +3047,19 @@ void launch_update(Launcher &L, int stage, int prim) {
   HIPCHK(hipGetLastError());
 }
 
-// the ideal gas's stage update (tendency only) with its tuned levels per thread
-void launch_update_na(Launcher &L, int kind, int kid, StageW sw) {
-  bool mix = kind == K_UPD_MIX_NA;
-  switch (L.shape[kind] ? L.shape[kind] : UPD_NA_L) {
-    case 2:
-      if (mix) launch(L, kind, kid, k_update_na<true, 2>, BLK, 0, sw);
-      else launch(L, kind, kid, k_update_na<false, 2>, BLK, 0, sw);
-      break;
-    case 4:
-      if (mix) launch(L, kind, kid, k_update_na<true, 4>, BLK, 0, sw);
-      else launch(L, kind, kid, k_update_na<false, 4>, BLK, 0, sw);
-      break;
-    default:
-      if (mix) launch(L, kind, kid, k_update_na<true, 1>, BLK, 0, sw);
-      else launch(L, kind, kid, k_update_na<false, 1>, BLK, 0, sw);
-  }
-}
-
-// the stage's update and column correction; the instances with the full
-// column solve leave the weighted average of u to it.  fused: the x2 faces
-// of the instances k_face2_update covers were left to it (launch_fluxes)
-void launch_update_implicit(Launcher &L, int stage, bool fused = false) {
+// the stage's update and column correction after launch_fluxes(L, true): the
+// ideal gas's x2 faces and tendency in one kernel (k_face2_update), whose
+// column step then forms the stage's weighted average of u; shallow water's
+// update (the new state and its primitives)
+void launch_update_implicit(Launcher &L, int stage) {
   int st0 = stage - 1;
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
             1, DFLOOR, SFLOOR};
   launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
-  if (fused) {
-    launch(L, K_FUSE2, KID_UPDATE, k_face2_update<false>, BLK, 0, sw);
-    launch(L, K_FUSE2_S, KID_UPDATE, k_face2_update<true>, BLK, 0, sw);
-  } else {
-    launch_update_na(L, K_UPD_NA, KID_UPDATE, sw);
-  }
-  launch_update_na(L, K_UPD_MIX_NA, KID_UPDATE, sw);
+  launch(L, K_FUSE2, KID_UPDATE, k_face2_update<false, false>, BLK, 0, sw);
+  launch(L, K_FUSE2_S, KID_UPDATE, k_face2_update<true, false>, BLK, 0, sw);
+  launch(L, K_FUSE2_MIX, KID_UPDATE, k_face2_update<false, true>, BLK, 0, sw);
+  launch(L, K_FUSE2_MIX_S, KID_UPDATE, k_face2_update<true, true>, BLK, 0, sw);
   launch(L, K_IMPL_AVG, KID_IMPLICIT, k_implicit<true>, 64, 0, sw);
   launch(L, K_IMPL3_AVG, KID_IMPLICIT, k_implicit3<true>, 64, 0, sw);
   launch(L, K_ADD_DU_AVG, KID_IMPLICIT, k_add_du<true>, BLK, 0, sw);
@@ -3089,7 +3096,7 @@ void launch_fill_ghosts_stage(Launcher &L) {
 
 void run_stage(Launcher &L, int stage) {
   launch_fluxes(L, true);
-  launch_update_implicit(L, stage, true);
+  launch_update_implicit(L, stage);
   launch_fill_ghosts_stage(L);
 }
 
@@ -3118,22 +3125,16 @@ void tune_launcher(Launcher &L, int reps) {
   HIPCHK(hipEventCreate(&e1));
   bool prof = L.prof;
   L.prof = false;
-  // the stage updates rewrite the scalars in place: their shape is tuned only
-  // when no instance carries scalars (then repeating one changes nothing)
-  bool scalars = false;
-  for (const Inst &in : L.h_inst) scalars = scalars || in.g.nt > 0;
-  // the updates read the face fluxes of the current state
+  // the fused x2-face updates read the x1 and x3 face fluxes of the current
+  // state, and those without scalars write only the tendency (the ones with
+  // scalars rewrite s in place and keep the plain launch)
   launch_fluxes(L);
   StageW sw{1.0, 0.0, 1.0, 1.0, 1, 0, DFLOOR, SFLOOR};
-  const int kinds[] = {K_PRIM,     K_GHOST_U, K_X1,          K_X1_MIX, K_EDGE,   K_EDGE_MIX,
-                       K_FACE,     K_FACE_MIX, K_FACE_MIX_SW, K_DT,     K_UPD_NA, K_UPD_MIX_NA,
-                       K_FUSE2};
+  const int kinds[] = {K_PRIM, K_GHOST_U,  K_X1,          K_X1_MIX, K_EDGE,  K_EDGE_MIX,
+                       K_FACE, K_FACE_MIX, K_FACE_MIX_SW, K_DT,     K_FUSE2, K_FUSE2_MIX};
   for (int kind : kinds) {
-    bool upd = kind == K_UPD_NA || kind == K_UPD_MIX_NA;
-    if (upd && scalars) continue;
     std::vector<int> shapes = {0};  // the default first
     if (kind == K_X1 || kind == K_X1_MIX) shapes = {0, 2, 8};
-    if (upd) shapes = {0, 2, 4};
     float best = 1e30f;
     int best_cap = 0, best_shape = 0;
     for (int sh : shapes) {
@@ -3162,8 +3163,12 @@ void tune_launcher(Launcher &L, int reps) {
             case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1, false>, BLK, 0); break;
             case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
             case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
-            case K_UPD_NA: case K_UPD_MIX_NA: launch_update_na(L, kind, -1, sw); break;
-            case K_FUSE2: launch(L, K_FUSE2, -1, k_face2_update<false>, BLK, 0, sw); break;
+            case K_FUSE2:
+              launch(L, K_FUSE2, -1, k_face2_update<false, false>, BLK, 0, sw);
+              break;
+            case K_FUSE2_MIX:
+              launch(L, K_FUSE2_MIX, -1, k_face2_update<false, true>, BLK, 0, sw);
+              break;
           }
         }
         HIPCHK(hipEventRecord(e1, L.stream));
@@ -3181,8 +3186,7 @@ void tune_launcher(Launcher &L, int reps) {
     L.cap[kind] = best_cap;
   }
   // kinds sharing a shape with a tuned one (the other update-shaped kinds keep
-  // the plain launch: they stream and cannot be repeated without changing the
-  // state)
+  // the plain launch: they rewrite the state and cannot be repeated)
   L.cap[K_GHOST_S] = L.cap[K_GHOST_U];
   L.cap[K_FACE_NF] = L.cap[K_FACE];
   L.prof = prof;

@@ -155,9 +155,10 @@ EXO_API int exo_profile_end(exo_handle *h, double *ms, int *counts, int nkinds);
 EXO_API int exo_conserved_sums(exo_handle *h, double *out5);
 /* Autotune the launches of the handle's kernels that only read the conserved
  * state or only write the tendency (repeated launches change nothing): the
- * persistent launch width, and the block shape where a kernel has several
- * (waves per 64-column block of the x1 sweep: 2, 4 or 8; levels per thread
- * of the stage update: 1, 2 or 4, tuned when no passive scalars ride along).
+ * persistent launch width (the face kernels, the fused x2-face stage update
+ * of instances without passive scalars, the primitive conversion, the ghost
+ * remap, the CFL minimum), and the x1 sweep's block shape (waves per
+ * 64-column block: 2, 4 or 8).
  * Each candidate is timed `reps` times and the fastest kept; a configuration
  * tuned before in this process reuses its result without timing.  caps (may
  * be NULL) receives up to ncaps values: the width of each launch kind (0 = one

@@ -80,10 +80,9 @@ def test_batch_persistent_and_tuned(ex):
     members = [make(ex, sp, SEED + q) for q, sp in enumerate(specs)]
     b = ex.Batch(members, graph=True)
     # a very narrow persistent launch: every workgroup loops over many blocks;
-    # the x1 sweeps and the stage updates in their non-default block shapes
+    # the x1 sweeps in their non-default block shapes
     caps = {k: 7 for k in ex.LAUNCH_KINDS}
-    caps.update({"x1_shape": 8, "x1_mix_shape": 2, "update_na_shape": 4,
-                 "update_mix_na_shape": 1})
+    caps.update({"x1_shape": 8, "x1_mix_shape": 2})
     b.set_caps(caps)
     b.step(2)
     caps = b.tune(reps=2)
