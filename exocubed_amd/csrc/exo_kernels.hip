@@ -350,10 +350,10 @@ __device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st
   else if (side == 1) { j = g.je + 1 + dq; k = g.ks + a; }
   else if (side == 2) { k = g.ks - 1 - dq; j = g.js + a; }
   else { k = g.ke + 1 + dq; j = g.js + a; }
+  // plain loads: the ghost remap has just written these cells
   double uv[5];
 #pragma unroll
-  for (int n = 0; n < 5; ++n)
-    uv[n] = g.skipv(n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
+  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
   prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
 }
 
