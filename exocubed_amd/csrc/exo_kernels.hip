@@ -819,15 +819,14 @@ __device__ __forceinline__ double stage_avg(const StageW &sw, double u, double u
 // NOAVG (ideal gas within a stage whose column solve follows): the weighted
 // average of u is left to the column solve's backward substitution, which
 // reads and writes u anyway; here only the tendency is formed.
-// NOSCAL: the instance has no passive scalars (the caller checked nt == 0),
-// so the cell's update stores nothing and a caller can form several cells'
-// updates with shared loads.
-// Face fluxes a caller holds in registers (FP64): the cell's x2 faces j and
-// j + 1 (x2l, x2r), its x1 faces i and i + 1 (x1l, x1r), and the scalars' x2
-// faces (s2l, s2r; up to 4 scalars).
+// NOSCAL: the instance has no passive scalars (the caller checked nt == 0).
+// Face fluxes a caller holds in registers: FP64, the cell's x2 faces j and
+// j + 1 (x2l, x2r), its x1 faces i and i + 1 (x1l, x1r) and the scalars' x2
+// faces (s2l, s2r; up to 4 scalars); mixed, the FP32 x2 faces of the cell and
+// of its scalars.
 struct RegFlux {
   const double *x2l, *x2r, *x1l, *x1r, *s2l, *s2r;
-  const float *f2l, *f2r, *fs2l, *fs2r;  // the mixed form's FP32 x2 face fluxes
+  const float *f2l, *f2r, *fs2l, *fs2r;
 };
 constexpr int RF_NT = 4;  // scalars register fluxes carry (the configuration's maximum)
 
@@ -1104,9 +1103,11 @@ __device__ __forceinline__ void face_flux_mix(const DGrid &g, const DState &st, 
                0.f);
 }
 
-// The x2 faces and the stage update (tendency only) of an FP64 ideal gas in
-// one pass, so the x2 face fluxes never go through memory; SCAL: with its
-// passive scalars (their x2 face fluxes too; the update averages s in place).
+// The x2 faces and the stage update (tendency only) of the ideal gas in one
+// pass, so the x2 face fluxes never go through memory: FP64, or MIX the mixed
+// form (FP32 faces of face_flux_mix, the update of update_cell<true>); SCAL:
+// with the passive scalars (their x2 face fluxes too; the update averages s
+// in place).
 // The pass writes du, s and s1, which no face of the launch reads (the faces
 // read w and r; a shallow-water update here would race: it writes the
 // primitive state the neighbouring wavefronts' face stencils read).  The
@@ -1115,7 +1116,8 @@ __device__ __forceinline__ void face_flux_mix(const DGrid &g, const DState &st, 
 // 63 w + l.  Lanes 0..62 then update the cell right of their face (its face
 // j), with face j + 1 from the next lane; a row's last face updates no cell.
 // Every face and every cell is covered once, with the arithmetic of
-// face_flux64<1>, flux_h_body<1>'s scalar fluxes and update_cell.
+// face_flux64<1> (or face_flux_mix<1>), the face kernels' scalar fluxes and
+// update_cell.
 constexpr int F2_FACES = 63;  // cells a wavefront updates (faces per wavefront - 1)
 
 template <bool SCAL, bool MIX>
