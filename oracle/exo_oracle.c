@@ -18,8 +18,11 @@
  *     (exo_oracle_math.c, pinned against libm by tests/test_oracle_math.py);
  *   - the 5x5 inverse and products (Eigen in the reference) use LU with partial
  *     pivoting and k-ordered sums (full_correction.cpp / forward_backward.hpp).
- * Parity unpinned (no reference test; restated from Athena++ conventions):
- *   AddFluxDivergence, SSPRK3 weighted averages, NewBlockTimeStep, reflecting BC.
+ * Pinned by physics (tests/oracle_checks.py, see exo_oracle.h): the values of
+ * AddFluxDivergence, the SSPRK3 weighted averages and NewBlockTimeStep.
+ * Parity unpinned (rounding order only; no reference test; restated from
+ * Athena++ conventions): the operation order inside AddFluxDivergence and the
+ * SSPRK3 average, Mesh::NewTimeStep's x2 limit, the reflecting-BC ghost fill.
  */
 #include "exo_oracle.h"
 
