@@ -565,7 +565,7 @@ def main():
                     help="only the 256-instance shallow-water batch, both forms (for profiling)")
     ap.add_argument("--only-serving", action="store_true",
                     help="only the serving stream at the default width, both forms")
-    ap.add_argument("--large-steps", type=int, default=10)
+    ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--implicit-flag", type=int, default=9, choices=(1, 9),
                     help="column correction of the headline workload: 9 full 5x5 (the "
                          "benchmark), 1 partial 3x3 (for profiling k_implicit3)")

@@ -65,6 +65,7 @@ struct GridT {
   // geometry
   PTR(const double) x1f, x1v, dx1f;
   PTR(const double) tab;               // [T_NTAB][6][nc3][nc2]
+  PTR(const double) tab1;              // [T1_NTAB][6][nc2 + 1] per-row tangent factors
   size_t t2stride;                     // 6*nc3*nc2
   __host__ __device__ size_t c(int v, int p, int k, int i, int j) const {
     return v * vstride + p * pstride + k * plane + (size_t)i * jp + j;
@@ -78,6 +79,10 @@ struct GridT {
   }
   __device__ double T(int t, int p, int k, int j) const {
     return tab[t * t2stride + ((size_t)p * nc3 + k) * nc2 + j];
+  }
+  // row q (j for the x2 factors, k for the x3 factors; the panel is square)
+  __device__ double T1(int t, int p, int q) const {
+    return tab1[((size_t)t * 6 + p) * (nc2 + 1) + q];
   }
   __device__ double area1(int p, int k, int j, int i) const {
     return T(T_DX2A, p, k, j) * T(T_DX3A, p, k, j) * x1f[i] * x1f[i] * T(T_SINC, p, k, j);
@@ -139,6 +144,7 @@ __device__ __forceinline__ DGrid to_global(const Grid &h) {
   g.plane = h.plane; g.pstride = h.pstride; g.vstride = h.vstride;
   g.fplane = h.fplane; g.fpstride = h.fpstride; g.fvstride = h.fvstride;
   g.x1f = glb(h.x1f); g.x1v = glb(h.x1v); g.dx1f = glb(h.dx1f); g.tab = glb(h.tab);
+  g.tab1 = glb(h.tab1);
   g.t2stride = h.t2stride;
   return g;
 }
@@ -228,8 +234,8 @@ __device__ __forceinline__ int xcd_order(int h, int n) {
 // ideal_moist_hydro.cpp): w, the reflecting x1 mirrors and the scalar mixing
 // ratios; the shallow-water depth floor is written back to u.
 __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int p, int k, int i,
-                                          int j, const double *uv, double dfloor, double sfloor) {
-  double cth = g.T(T_COSC, p, k, j);
+                                          int j, const double *uv, double dfloor, double sfloor,
+                                          double cth) {
   double wv[5] = {0., 0., 0., 0., 0.};
   double rho_c;
   if (g.eos == EOS_SW) {
@@ -326,7 +332,7 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
 #pragma unroll
   for (int n = 0; n < 5; ++n)
     uv[n] = g.skipv(n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
-  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
+  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
 }
 
 // The ghost strips only (3 deep along each panel edge, corners excluded): the
@@ -354,7 +360,7 @@ __device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st
   double uv[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
-  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor);
+  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
 }
 
 __global__ void __launch_bounds__(BLK) k_prim(const Inst *__restrict__ I_, Work w, double dfloor, double sfloor) {
@@ -833,10 +839,12 @@ constexpr int RF_NT = 4;  // scalars register fluxes carry (the configuration's 
 // X2R: the fluxes of rf come from the caller's registers instead of the flux
 // arrays (FP64: x2, x1 and the scalars' x2; mixed: the FP32 x2 and scalar x2
 // faces).
+// cth_out (if not null) receives the cell's cos(theta).
 template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out,
-                                            const RegFlux *rf = nullptr) {
+                                            const RegFlux *rf = nullptr,
+                                            double *cth_out = nullptr) {
   double dt = st.dtv[0];
   double wght = sw.beta * dt;
   bool tend = g.eos == EOS_IDEAL;
@@ -847,6 +855,18 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   }
   double a2l = g.area2(p, k, j, i), a2r = g.area2(p, k, j + 1, i);
   double a3l = g.area3(p, k, j, i), a3r = g.area3(p, k + 1, j, i);
+  // Shallow water has one layer, so per-cell tables are read once per cell (a
+  // 3-D column reads them once for all its levels).  Its tangents x, y come
+  // from the panel's per-row tables instead, and cos theta = -x y / (C D)
+  // from them with the set-up table's expression (exo_setup.cpp), which gives
+  // the table's bits for one quotient.
+  const bool rows = !X2R && g.eos == EOS_SW;
+  double rx = 0., ry = 0., rcos = 0.;
+  if (rows) {
+    rx = g.T1(T1_X, p, j);
+    ry = g.T1(T1_Y, p, k);
+    rcos = qdiv(-rx * ry, g.T1(T1_C, p, j) * g.T1(T1_D, p, k));
+  }
   double vol = g.volume(p, k, j, i);
   // every quotient by vol (and by the radius below) shares one correctly
   // rounded reciprocal: div_rn gives the IEEE quotient
@@ -915,7 +935,9 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
   }
   // coordinate source terms
-  double cosine = g.T(T_COSC, p, k, j), sinc = g.T(T_SINC, p, k, j);
+  double cosine = rows ? rcos : g.T(T_COSC, p, k, j);
+  double sinc = g.T(T_SINC, p, k, j);
+  if (cth_out) *cth_out = cosine;
   double sine2 = sinc * sinc;
   // the shallow-water x1 velocity is zero for good (not loaded)
   double v1 = g.eos == EOS_SW ? 0. : st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
@@ -945,7 +967,8 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     out[IEN] += src * v1;
   }
   if (g.coriolis != 0) {
-    double X = g.T(T_TANX, p, k, j), Y = g.T(T_TANY, p, k, j), f = g.T(T_CORF, p, k, j);
+    double X = rows ? rx : g.T(T_TANX, p, k, j), Y = rows ? ry : g.T(T_TANY, p, k, j);
+    double f = g.T(T_CORF, p, k, j);
     double U, V, acc2, acc3;
     vec_rll_from_abp(X, Y, p + 1, v2, v3, &U, &V);  // CubedSphere::GetUV
     U = -U;
@@ -959,8 +982,8 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
       llV = -f * m2;
     }
     vec_abp_from_rll(X, Y, p + 1, -llU, llV, &acc2, &acc3);  // GetVyVz
-    double C = sqrt(X * X + 1), D = sqrt(Y * Y + 1);
-    double cth = qdiv(-X * Y, C * D);
+    // (with the row factors: the same C, D and quotient as the cosine above)
+    double cth = rows ? rcos : qdiv(-X * Y, sqrt(X * X + 1) * sqrt(Y * Y + 1));
     double a2c = acc2 + acc3 * cth, a3c = acc3 + acc2 * cth;
     if (g.coriolis == 1) {
       out[IVY] += wght * rho * a2c;
@@ -971,11 +994,9 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
   }
   // passive scalars: weighted average and flux divergence into s (with
-  // register fluxes the loop runs over the largest count, so the flux arrays
-  // are indexed by constants and stay in registers)
-#pragma unroll
-  for (int n = 0; n < (NOSCAL ? 0 : X2R ? RF_NT : g.nt); ++n) {
-    if (X2R && n >= g.nt) break;
+  // register fluxes the loop is unrolled over the largest count, so the flux
+  // arrays are indexed by constants and stay in registers)
+  auto scalar = [&](int n) {
     size_t c = g.c(n, p, k, i, j);
     double s = st.s[c], s1;
     if (sw.first) {
@@ -1010,6 +1031,14 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
     s -= div_rn(wght * dflx, vol, yvol);
     st.s[c] = s;
+  };
+  if constexpr (NOSCAL) {
+  } else if constexpr (X2R) {
+#pragma unroll
+    for (int n = 0; n < RF_NT; ++n)
+      if (n < g.nt) scalar(n);
+  } else {
+    for (int n = 0; n < g.nt; ++n) scalar(n);
   }
 }
 
@@ -1029,13 +1058,14 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   r /= ni;
   int k = g.ks + (int)(r % N), p = (int)(r / N);
   double out[5];
-  update_cell<MIX, NOAVG>(g, st, p, k, i, j, sw, out);
+  double cth;
+  update_cell<MIX, NOAVG>(g, st, p, k, i, j, sw, out, nullptr, &cth);
   ADDR(1) double *dst = g.eos == EOS_IDEAL ? st.du : st.u;
   for (int n = 0; n < g.nv; ++n)
     if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
   // shallow water within a step: the stage's new state is final here, so its
   // primitive state is written now (k_prim_ghost then covers the ghosts)
-  if (sw.prim && g.eos == EOS_SW) prim_from(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor);
+  if (sw.prim && g.eos == EOS_SW) prim_from(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor, cth);
 }
 
 template <bool MIX, bool NOAVG = false>
@@ -2884,7 +2914,7 @@ size_t footprint_bytes(const Setup &s) {
   size_t cell = g.vstride * sizeof(double), face = g.fvstride * sizeof(double);
   size_t edge = (size_t)6 * 4 * g.N * g.nx1;
   add(3 * (s.nc1 + 1) * sizeof(double));
-  add(s.tab2.size() * sizeof(double));
+  add((s.tab2.size() + s.tab1.size()) * sizeof(double));
   add(s.ghosts.size() * sizeof(GhostDesc));
   add(s.edges.size() * sizeof(EdgeDesc));
   for (int q = 0; q < 4; ++q) add(cell * g.nv);           // u, w, u1, du
@@ -2937,10 +2967,14 @@ void init_handle(exo_handle *h) {
   g.x1f = h->d_x1;
   g.x1v = h->d_x1 + (s.nc1 + 1);
   g.dx1f = h->d_x1 + 2 * (s.nc1 + 1);
-  h->d_tab = (double *)h->dalloc(s.tab2.size() * sizeof(double));
+  // the 2-D tables, then the per-row ones
+  h->d_tab = (double *)h->dalloc((s.tab2.size() + s.tab1.size()) * sizeof(double));
   HIPCHK(hipMemcpyAsync(h->d_tab, s.tab2.data(), s.tab2.size() * sizeof(double),
                         hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->d_tab + s.tab2.size(), s.tab1.data(), s.tab1.size() * sizeof(double),
+                        hipMemcpyHostToDevice, h->stream));
   g.tab = h->d_tab;
+  g.tab1 = h->d_tab + s.tab2.size();
   h->d_ghost = (GhostDesc *)h->dalloc(s.ghosts.size() * sizeof(GhostDesc));
   HIPCHK(hipMemcpyAsync(h->d_ghost, s.ghosts.data(), s.ghosts.size() * sizeof(GhostDesc),
                         hipMemcpyHostToDevice, h->stream));
@@ -3584,6 +3618,8 @@ int exo_stage(exo_handle *h, int stage, double dt) {
   });
 }
 
+}  // extern "C"
+
 namespace {
 // the step graph of a handle (captured on first use, rebuilt when the dt mode
 // or the launch widths change); kernel nodes only
@@ -3681,6 +3717,7 @@ uint64_t fnv64(const double *a, size_t n, uint64_t hsh) {
   return hsh;
 }
 }  // namespace
+extern "C" {
 
 int exo_step(exo_handle *h, int nsteps, double dt_fixed, double *dt_last) {
   return guarded(h, [&] {
@@ -4027,10 +4064,6 @@ int exo_init_state_cfg(const exo_config *cfg, int kind, uint64_t seed, const dou
   return EXO_OK;
 }
 
-// Host-only introspection (no device needed): the set-up tables a handle
-// would upload.  which: 0 ghost descriptors, 1 edge descriptors, 2 metric
-// tables [T_NTAB][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f of the six panels,
-// 5 x3f of the six panels.
 // ---------------------------------------------------------------------------
 // output (NetcdfOutput::WriteOutputFile, src/outputs/netcdf.cpp:34-469)
 int exo_output_names(const exo_config *cfg, const char *variable, char *names, size_t cap,
@@ -4100,6 +4133,10 @@ int exo_write_netcdf_fields(const exo_config *cfg, const char *basename, const c
                                                                          : EXO_ERR_ARG;
 }
 
+// Host-only introspection (no device needed): the set-up tables a handle
+// would upload.  which: 0 ghost descriptors, 1 edge descriptors, 2 metric
+// tables [T_NTAB][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f of the six panels,
+// 5 x3f of the six panels.
 int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t *count) {
   if (!cfg || !count) return EXO_ERR_ARG;
   Setup s;
@@ -4150,6 +4187,8 @@ int exo_conserved_sums(exo_handle *h, double *out5) {
     HIPCHK(hipStreamSynchronize(h->stream));
   });
 }
+
+}  // extern "C"
 
 namespace {
 static_assert(K_NKIND == EXO_NKINDS, "EXO_NKINDS must count the launch kinds");
@@ -4205,6 +4244,7 @@ void report_tuning(const Launcher &L, int *caps, int ncaps) {
     caps[q] = q < K_NKIND ? L.cap[q] : L.shape[q - K_NKIND];
 }
 }  // namespace
+extern "C" {
 
 int exo_tune(exo_handle *h, int reps, int *caps, int ncaps) {
   return guarded(h, [&] {
@@ -4219,6 +4259,8 @@ int exo_tune(exo_handle *h, int reps, int *caps, int ncaps) {
 }
 
 // --- batches ---------------------------------------------------------------
+}  // extern "C"
+
 namespace {
 void batch_free_graph(exo_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -4234,6 +4276,7 @@ void batch_ready(exo_batch *b) {
   }
 }
 }  // namespace
+extern "C" {
 
 int exo_batch_create(exo_handle *const *members, int n, int flags, exo_batch **out) {
   if (!members || n <= 0 || !out) return EXO_ERR_ARG;

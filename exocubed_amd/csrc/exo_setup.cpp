@@ -368,6 +368,7 @@ std::string build_setup(const exo_config &cfg, Setup *s) {
   for (int i = 0; i < nc1; ++i) s->dx1f[i] = dx1;
   for (int i = 0; i < nc1; ++i) s->x1v[i] = 0.5 * (s->x1f[i + 1] + s->x1f[i]);
   s->tab2.assign((size_t)T_NTAB * 6 * nc3 * nc2, 0.);
+  s->tab1.assign((size_t)T1_NTAB * 6 * (nc2 + 1), 0.);
   for (int p = 0; p < 6; ++p) {
     int lx2 = p % 2, lx3 = p / 2;
     auto &x2f = s->x2f[p], &x2v = s->x2v[p], &x3f = s->x3f[p], &x3v = s->x3v[p];
@@ -381,17 +382,31 @@ std::string build_setup(const exo_config &cfg, Setup *s) {
     for (int k = 0; k <= nc3; ++k)
       x3f[k] = panel_x3(mesh_fraction((long)(k - NG) + (long)lx3 * N, 3L * N), lx3);
     for (int k = 0; k < nc3; ++k) x3v[k] = 0.5 * (x3f[k + 1] + x3f[k]);
+    double *T1 = s->tab1.data();
+    for (int q = 0; q <= nc2; ++q) {
+      double xf = tan(x2f[q]), yf = tan(x3f[q]);
+      T1[s->t1(T1_XF, p, q)] = xf;
+      T1[s->t1(T1_CF, p, q)] = sqrt(1.0 + xf * xf);
+      T1[s->t1(T1_YF, p, q)] = yf;
+      T1[s->t1(T1_DF, p, q)] = sqrt(1.0 + yf * yf);
+      if (q == nc2) break;
+      double x = tan(x2v[q]), y = tan(x3v[q]);
+      T1[s->t1(T1_X, p, q)] = x;
+      T1[s->t1(T1_C, p, q)] = sqrt(1.0 + x * x);
+      T1[s->t1(T1_Y, p, q)] = y;
+      T1[s->t1(T1_D, p, q)] = sqrt(1.0 + y * y);
+    }
     // GnomonicEquiangle constructor, gnomonic_equiangle.cpp:154-196
     for (int k = 0; k < nc3; ++k)
       for (int j = 0; j < nc2; ++j) {
-        double x = tan(x2v[j]);
-        double xf = tan(x2f[j]);
-        double y = tan(x3v[k]);
-        double yf = tan(x3f[k]);
-        double C = sqrt(1.0 + x * x);
-        double Cf = sqrt(1.0 + xf * xf);
-        double D = sqrt(1.0 + y * y);
-        double Df = sqrt(1.0 + yf * yf);
+        double x = T1[s->t1(T1_X, p, j)];
+        double xf = T1[s->t1(T1_XF, p, j)];
+        double y = T1[s->t1(T1_Y, p, k)];
+        double yf = T1[s->t1(T1_YF, p, k)];
+        double C = T1[s->t1(T1_C, p, j)];
+        double Cf = T1[s->t1(T1_CF, p, j)];
+        double D = T1[s->t1(T1_D, p, k)];
+        double Df = T1[s->t1(T1_DF, p, k)];
         double *T = s->tab2.data();
         T[s->t2(T_COSC, p, k, j)] = -x * y / (C * D);
         T[s->t2(T_SINC, p, k, j)] = sqrt(1.0 + x * x + y * y) / (C * D);

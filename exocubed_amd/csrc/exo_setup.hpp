@@ -38,6 +38,12 @@ enum Tab2 {
   T_NTAB
 };
 
+// 1-D (per panel, rows 0..nc2) tangent factors of the gnomonic coordinates:
+// x = tan(x2v(j)), xf = tan(x2f(j)), y = tan(x3v(k)), yf = tan(x3f(k)) and
+// C = sqrt(1 + x^2), CF, D = sqrt(1 + y^2), DF; the 2-D tables are formed
+// from them (the cell-centre entries of the last row are unused)
+enum Tab1 { T1_X = 0, T1_XF, T1_C, T1_CF, T1_Y, T1_YF, T1_D, T1_DF, T1_NTAB };
+
 // One face-ghost cell of the panel-edge remap (cubed_sphere_utility.cpp:92-369
 // as packed by PackData and unpacked by SetBoundarySameLevel).
 struct GhostDesc {
@@ -73,9 +79,11 @@ struct Setup {
   std::vector<double> x1f, x1v, dx1f;   // shared by all panels
   std::vector<double> x2f[6], x2v[6], x3f[6], x3v[6];
   std::vector<double> tab2;             // [T_NTAB][6][nc3][nc2]
+  std::vector<double> tab1;             // [T1_NTAB][6][nc2 + 1]
   std::vector<GhostDesc> ghosts;        // 6 panels * 4 edges * 3 rows * N
   std::vector<EdgeDesc> edges;          // 6 panels * 4 edges * N
 
+  size_t t1(int t, int p, int q) const { return ((size_t)t * 6 + p) * (nc2 + 1) + q; }
   size_t t2(int t, int p, int k, int j) const {
     return (((size_t)t * 6 + p) * nc3 + k) * nc2 + j;
   }
