@@ -51,6 +51,22 @@ EXO_HD double qdiv(double x, double d) {
 EXO_HD float qdiv(float x, float d) { return x / d; }
 
 // ---------------------------------------------------------------------------
+// Correctly rounded quotient x / d from y = RN(1/d), one IEEE division shared
+// by every quotient with the same divisor.  q0 = RN(x y) lies within one ulp
+// of x/d, so the remainder x - q0 d is exact under FMA and RN(q0 + r y) is the
+// correctly rounded x/d (Markstein's theorem, IBM J. Res. Dev. 34(1), 1990;
+// the basis of the IA-64 division sequences).  It needs x/d, q0 and r away
+// from overflow and the subnormal range, which holds for the geometric and
+// thermodynamic quantities divided here.  Three instructions instead of the
+// ~10 of an IEEE division, with the identical result (one exception: x = -0
+// gives +0, an equal value, where IEEE division keeps the sign).
+EXO_HD double div_rn(double x, double d, double y) {
+  double q0 = x * y;
+  double r = fma(-q0, d, x);
+  return fma(r, y, q0);
+}
+
+// ---------------------------------------------------------------------------
 // Deterministic exp / log / pow (FP64) used by the hydrostatic decomposition
 // (change_to_perturbation.cpp:63-65,109-115,181-182; decomposition.cpp:357-360).
 // Cody-Waite reduction with a degree-13 Taylor polynomial for exp, and the
@@ -411,20 +427,6 @@ EXO_HD void riemann(int eos, int rs, int ivx, R gamma, R *wl, R *wr, R *f, R cth
   }
 }
 
-// ---------------------------------------------------------------------------
-// Correctly rounded quotient x / d from y = RN(1/d), one IEEE division shared
-// by every quotient with the same divisor.  q0 = RN(x y) lies within one ulp
-// of x/d, so the remainder x - q0 d is exact under FMA and RN(q0 + r y) is the
-// correctly rounded x/d (Markstein's theorem, IBM J. Res. Dev. 34(1), 1990;
-// the basis of the IA-64 division sequences).  It needs x/d, q0 and r away
-// from overflow and the subnormal range, which holds for the geometric and
-// thermodynamic quantities divided here.  Three instructions instead of the
-// ~10 of an IEEE division, with the identical result.
-EXO_HD double div_rn(double x, double d, double y) {
-  double q0 = x * y;
-  double r = fma(-q0, d, x);
-  return fma(r, y, q0);
-}
 
 // ---------------------------------------------------------------------------
 // 5x5 algebra of the implicit column solve (Eigen in full_correction.cpp):

@@ -302,7 +302,7 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
     }
   }
   for (int n = 0; n < g.nt; ++n) {
-    double rr = st.s[g.c(n, p, k, i, j)] * (1.0 / rho_c);
+    double rr = st.s[g.c(n, p, k, i, j)] * qdiv(1.0, rho_c);
     rr = rr < sfloor ? sfloor : rr;
     st.r[g.c(n, p, k, i, j)] = rr;
     if (qb <= NG) st.r[g.c(n, p, k, g.is - qb, j)] = rr;
@@ -2350,7 +2350,7 @@ __global__ void __launch_bounds__(BLK) k_edge_states_mix(const Inst *__restrict_
 template <int dir>
 __device__ __forceinline__ void sw_base_flux(double h, double v2, double v3, double cf, double sf,
                                              double ct, double *f) {
-  double hs = h * sf, Ps = 0.5 * h * h / sf;
+  double hs = h * sf, Ps = qdiv(0.5 * h * h, sf);
   double fy, fz;
   if (dir == 1) {
     f[IDN] = hs * v2;
