@@ -405,21 +405,29 @@ EXO_HD void lmars(int ivx, R gamma, const R *wli, const R *wri, R *fl) {
 }
 
 // Hydro::RiemannSolver for one face: frame transforms around the solver as
-// each solver file does them.  wl / wr are modified (local frame).
+// each solver file does them.  wl / wr are modified (local frame).  The
+// configuration pairs shallow water with the Roe or LMARS shallow-water
+// solver and the ideal gas with HLLC or LMARS, so a caller that fixes eos at
+// compile time compiles only its two solvers.
 template <typename R>
 EXO_HD void riemann(int eos, int rs, int ivx, R gamma, R *wl, R *wr, R *f, R cth, R cosf,
                     R sinf, R cosc) {
   f[0] = f[1] = f[2] = f[3] = R(0.);
   if (eos == EOS_IDEAL) f[4] = R(0.);
-  if (ivx != IVX && rs != RS_LMARS) {
+  bool maps = eos == EOS_SW || rs != RS_LMARS;  // lmars.cpp maps nothing
+  if (ivx != IVX && maps) {
     local_frame(ivx, cosf, sinf, wl);
     local_frame(ivx, cosf, sinf, wr);
   }
-  if (rs == RS_ROE_SW) roe_sw(ivx, wl, wr, f);
-  else if (rs == RS_LMARS_SW) lmars_sw(ivx, wl, wr, f);
-  else if (rs == RS_HLLC) hllc(ivx, gamma, wl, wr, f, cosc);
-  else lmars(ivx, gamma, wl, wr, f);
-  if (rs != RS_LMARS) {
+  if (eos == EOS_SW) {
+    if (rs == RS_ROE_SW) roe_sw(ivx, wl, wr, f);
+    else lmars_sw(ivx, wl, wr, f);
+  } else if (rs == RS_HLLC) {
+    hllc(ivx, gamma, wl, wr, f, cosc);
+  } else {
+    lmars(ivx, gamma, wl, wr, f);
+  }
+  if (maps) {
     if (ivx != IVX) flux_to_global(ivx, cosf, sinf, f);
     R ty = f[IVY], tz = f[IVZ];
     f[IVY] = ty + tz * cth;

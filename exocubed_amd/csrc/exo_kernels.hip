@@ -229,16 +229,33 @@ __device__ __forceinline__ int xcd_order(int h, int n) {
 // ghost it mirrors into (ghost is - q mirrors is + q - 1, ghost ie + q mirrors
 // ie - q + 1).  With nx1 < NG the ghosts beyond the mirror of an interior cell
 // are not written (only shallow water allows it, and it never reads x1 ghosts).
+// The equation of state of a kernel specialised for one (EK = EOS_SW or
+// EOS_IDEAL: the compiler drops the other's code and registers) or read from
+// the instance (EK < 0), and whether it skips variable n.
+template <int EK>
+__device__ __forceinline__ int eos_of(const DGrid &g) {
+  return EK >= 0 ? EK : g.eos;
+}
+template <int EK>
+__device__ __forceinline__ bool skip_of(const DGrid &g, int n) {
+  return EK == EOS_SW ? n == IVX || n > IVZ : EK == EOS_IDEAL ? false : g.skipv(n);
+}
+template <int EK>
+__device__ __forceinline__ int nv_of(const DGrid &g) {
+  return EK == EOS_SW ? 4 : EK == EOS_IDEAL ? 5 : g.nv;
+}
+
 // Primitive state of one cell (p, k, i, j) from its conserved values uv
 // (EquationOfState::ConservedToPrimitive of shallow_yz_hydro.cpp /
 // ideal_moist_hydro.cpp): w, the reflecting x1 mirrors and the scalar mixing
 // ratios; the shallow-water depth floor is written back to u.
+template <int EK = -1>
 __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int p, int k, int i,
                                           int j, const double *uv, double dfloor, double sfloor,
                                           double cth) {
   double wv[5] = {0., 0., 0., 0., 0.};
   double rho_c;
-  if (g.eos == EOS_SW) {
+  if (eos_of<EK>(g) == EOS_SW) {
     double ud = uv[IDN];
     if (!(ud > dfloor)) {  // the floor, written back only where it applies
       ud = dfloor;
@@ -281,10 +298,10 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
   }
   // mirrors into ghosts is - qb, ie + qt (shallow water never reads x1 ghosts)
   int qb = i - g.is + 1, qt = g.ie - i + 1;
-  bool mirror = g.eos != EOS_SW;
+  bool mirror = eos_of<EK>(g) != EOS_SW;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (g.skipv(n)) continue;
+    if (skip_of<EK>(g, n)) continue;
     st.w[g.c(n, p, k, i, j)] = wv[n];
     double m = n == IVX ? -wv[n] : wv[n];
     if (mirror && qb <= NG) st.w[g.c(n, p, k, g.is - qb, j)] = m;
@@ -315,6 +332,7 @@ __device__ __forceinline__ void prim_from(const DGrid &g, const DState &st, int 
 // u is loaded non-temporally: nothing reads it again before the stage's
 // column step, and the caches are left to w, which the flux kernels and the
 // CFL minimum read next (k_dt 0.055 -> 0.044 ms on 6 x 96 x 96 x 64)
+template <int EK = -1>
 __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int lb, double dfloor,
                                           double sfloor) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
@@ -331,13 +349,14 @@ __device__ __forceinline__ void prim_body(const DGrid &g, const DState &st, int 
   double uv[5];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    uv[n] = g.skipv(n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
-  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
+    uv[n] = skip_of<EK>(g, n) ? 0. : __builtin_nontemporal_load(&st.u[g.c(n, p, k, i, j)]);
+  prim_from<EK>(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
 }
 
 // The ghost strips only (3 deep along each panel edge, corners excluded): the
 // primitive state of the interior cells of shallow water is written by the
-// stage's update.  Threads cover [p][side][depth][along][i].
+// stage's update (shallow-water instances only).  Threads cover
+// [p][side][depth][along][i].
 __device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st, int lb,
                                                 double dfloor, double sfloor) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
@@ -359,14 +378,17 @@ __device__ __forceinline__ void prim_ghost_body(const DGrid &g, const DState &st
   // plain loads: the ghost remap has just written these cells
   double uv[5];
 #pragma unroll
-  for (int n = 0; n < 5; ++n) uv[n] = g.skipv(n) ? 0. : st.u[g.c(n, p, k, i, j)];
-  prim_from(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
+  for (int n = 0; n < 5; ++n) uv[n] = skip_of<EOS_SW>(g, n) ? 0. : st.u[g.c(n, p, k, i, j)];
+  prim_from<EOS_SW>(g, st, p, k, i, j, uv, dfloor, sfloor, g.T(T_COSC, p, k, j));
 }
 
-__global__ void __launch_bounds__(BLK) k_prim(const Inst *__restrict__ I_, Work w, double dfloor, double sfloor) {
+// EK: the equation of state of every instance of the launch, or -1 (any)
+template <int EK = -1>
+__global__ void __launch_bounds__(BLK) k_prim(const Inst *__restrict__ I_, Work w, double dfloor,
+                                              double sfloor) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    prim_body(g, st, lb, dfloor, sfloor);
+    prim_body<EK>(g, st, lb, dfloor, sfloor);
   }
 }
 
@@ -540,7 +562,7 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
       wl[IDN] += dsf;
       if (wl[IDN] < 0.) wl[IDN] = dsf;
     }
-    riemann<double>(g.eos, g.rs, IVX, g.gamma, wl, wr, fl, cth, 0., 1., cosc);
+    riemann<double>(EOS_IDEAL, g.rs, IVX, g.gamma, wl, wr, fl, cth, 0., 1., cosc);
 #pragma unroll
     for (int n = 0; n < 5; ++n)
       if (n < nv) st.flx[0][g.f(n, p, k, fi, j)] = fl[n];
@@ -641,14 +663,14 @@ __global__ void __launch_bounds__(BLK) k_edge_states(const Inst *__restrict__ I_
 // Hydro flux of the x2 (dir 1) or x3 (dir 2) face (p, k, i, j), j = js + fa
 // (x2) or k = ks + fa (x3), along-edge index kk: WENO5 L/R states, the
 // neighbour panel's state on a panel edge (fa = 0 or N), then riemann().
-template <int dir>
+template <int dir, int EK = -1>
 __device__ __forceinline__ void face_flux64(const DGrid &g, const DState &st, int p, int kk, int fa,
                                             int i, int k, int j, double *fl) {
   int N = g.N, ni = g.nx1;
   double wl[5] = {0, 0, 0, 0, 0}, wr[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (g.skipv(n)) continue;
+    if (skip_of<EK>(g, n)) continue;
     double m3, m2, m1, c0, p1, p2;
     if (dir == 1) {
       m3 = st.w[g.c(n, p, k, i, j - 3)];
@@ -669,18 +691,20 @@ __device__ __forceinline__ void face_flux64(const DGrid &g, const DState &st, in
     if (fa != 0) wl[n] = weno5(p1, c0, m1, m2, m3);
     if (fa != N) wr[n] = weno5(m2, m1, c0, p1, p2);
   }
-  size_t vs = (size_t)6 * 4 * N * ni;
-  int lo_side = dir == 1 ? 0 : 2;
-  if (fa == 0) {  // ghost side is the L state
-    size_t b = (((size_t)p * 4 + lo_side) * N + kk) * ni + (i - g.is);
+  // on a panel edge (fa = 0: the L state, fa = N: the R state) the ghost
+  // side is the exchanged state; the values are selected, not the array they
+  // go to, which keeps wl and wr in registers
+  if (fa == 0 || fa == N) {
+    size_t vs = (size_t)6 * 4 * N * ni;
+    int side = (dir == 1 ? 0 : 2) + (fa == 0 ? 0 : 1);
+    size_t b = (((size_t)p * 4 + side) * N + kk) * ni + (i - g.is);
 #pragma unroll
-    for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wl[n] = st.edge[n * vs + b];
-  } else if (fa == N) {  // ghost side is the R state
-    size_t b = (((size_t)p * 4 + lo_side + 1) * N + kk) * ni + (i - g.is);
-#pragma unroll
-    for (int n = 0; n < 5; ++n)
-      if (!g.skipv(n)) wr[n] = st.edge[n * vs + b];
+    for (int n = 0; n < 5; ++n) {
+      if (skip_of<EK>(g, n)) continue;
+      double e = st.edge[n * vs + b];
+      wl[n] = fa == 0 ? e : wl[n];
+      wr[n] = fa == 0 ? wr[n] : e;
+    }
   }
   double cosf, sinf, cth;
   if (dir == 1) {
@@ -692,10 +716,10 @@ __device__ __forceinline__ void face_flux64(const DGrid &g, const DState &st, in
     sinf = g.T(T_SIN3, p, k, j);
     cth = g.T(T_CTH3, p, k, j);
   }
-  riemann<double>(g.eos, g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
+  riemann<double>(eos_of<EK>(g), g.rs, dir == 1 ? IVY : IVZ, g.gamma, wl, wr, fl, cth, cosf, sinf, 0.);
 }
 
-template <int dir>
+template <int dir, int EK = -1>
 __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, int lb) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   int N = g.N, ni = g.nx1;
@@ -724,11 +748,11 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
   if (dir == 1) { k = g.ks + kk; j = g.js + fa; }
   else { k = g.ks + fa; j = g.js + kk; }
   double fl[5];
-  face_flux64<dir>(g, st, p, kk, fa, i, k, j, fl);
+  face_flux64<dir, EK>(g, st, p, kk, fa, i, k, j, fl);
   ADDR(1) double *F = st.flx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
+    if (!skip_of<EK>(g, n)) F[g.f(n, p, k, i, j)] = fl[n];
   for (int n = 0; n < g.nt; ++n) {
     double r[6];
     for (int s = 0; s < 6; ++s)
@@ -741,13 +765,14 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
 }
 
 // five wavefronts per SIMD: 96 VGPRs without spills, 4 % faster than the
-// compiler's four on the x3 faces of 6 x 96 x 96 x 64
-template <int dir>
+// compiler's four on the x3 faces of 6 x 96 x 96 x 64.  EK: the equation of
+// state of every instance of the launch, or -1 (any).
+template <int dir, int EK = -1>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5)))
 k_flux_h(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    flux_h_body<dir>(g, st, lb);
+    flux_h_body<dir, EK>(g, st, lb);
   }
 }
 
@@ -769,12 +794,12 @@ __device__ __forceinline__ void edge_flux_body(const DGrid &g, const DState &st,
   double fl[5];
   int dir = side < 2 ? 1 : 2;
   int k = dir == 1 ? g.ks + kk : g.ks + fa, j = dir == 1 ? g.js + fa : g.js + kk;
-  if (dir == 1) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
-  else face_flux64<2>(g, st, p, kk, fa, i, k, j, fl);
+  if (dir == 1) face_flux64<1, EOS_SW>(g, st, p, kk, fa, i, k, j, fl);
+  else face_flux64<2, EOS_SW>(g, st, p, kk, fa, i, k, j, fl);
   ADDR(1) double *F = st.flx[dir];
 #pragma unroll
   for (int n = 0; n < 5; ++n)
-    if (!g.skipv(n)) F[g.f(n, p, k, i, j)] = fl[n];
+    if (!skip_of<EOS_SW>(g, n)) F[g.f(n, p, k, i, j)] = fl[n];
 }
 
 __global__ void __launch_bounds__(BLK) k_edge_flux(const Inst *__restrict__ I_, Work w) {
@@ -840,16 +865,16 @@ constexpr int RF_NT = 4;  // scalars register fluxes carry (the configuration's 
 // arrays (FP64: x2, x1 and the scalars' x2; mixed: the FP32 x2 and scalar x2
 // faces).
 // cth_out (if not null) receives the cell's cos(theta).
-template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false>
+template <bool MIX, bool NOAVG = false, bool NOSCAL = false, bool X2R = false, int EK = -1>
 __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, int p, int k, int i,
                                             int j, StageW sw, double *out,
                                             const RegFlux *rf = nullptr,
                                             double *cth_out = nullptr) {
   double dt = st.dtv[0];
   double wght = sw.beta * dt;
-  bool tend = g.eos == EOS_IDEAL;
+  bool tend = eos_of<EK>(g) == EOS_IDEAL;
   double a1l = 0., a1r = 0.;  // x1 faces: shallow water has no x1 flux
-  if (g.eos != EOS_SW) {
+  if (eos_of<EK>(g) != EOS_SW) {
     a1l = g.area1(p, k, j, i);
     a1r = g.area1(p, k, j, i + 1);
   }
@@ -860,7 +885,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   // from the panel's per-row tables instead, and cos theta = -x y / (C D)
   // from them with the set-up table's expression (exo_setup.cpp), which gives
   // the table's bits for one quotient.
-  const bool rows = !X2R && g.eos == EOS_SW;
+  const bool rows = !X2R && eos_of<EK>(g) == EOS_SW;
   double rx = 0., ry = 0., rcos = 0.;
   if (rows) {
     rx = g.T1(T1_X, p, j);
@@ -877,7 +902,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   bool keep_u = sw.first && sw.delta == 1.0;
 #pragma unroll
   for (int n = 0; n < 5; ++n) {
-    if (g.skipv(n)) continue;
+    if (skip_of<EK>(g, n)) continue;
     size_t c = g.c(n, p, k, i, j);
     double u = 0., u1;
     if (NOAVG) {
@@ -898,7 +923,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     }
     double dflx;
     if (!MIX) {
-      if (g.eos == EOS_SW) dflx = 0.;
+      if (eos_of<EK>(g) == EOS_SW) dflx = 0.;
       else
         dflx = a1r * (X2R ? rf->x1r[n] : st.flx[0][g.f(n, p, k, i + 1, j)]) -
                a1l * (X2R ? rf->x1l[n] : st.flx[0][g.f(n, p, k, i, j)]);
@@ -940,13 +965,13 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   if (cth_out) *cth_out = cosine;
   double sine2 = sinc * sinc;
   // the shallow-water x1 velocity is zero for good (not loaded)
-  double v1 = g.eos == EOS_SW ? 0. : st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
+  double v1 = eos_of<EK>(g) == EOS_SW ? 0. : st.w[g.c(IVX, p, k, i, j)], v2 = st.w[g.c(IVY, p, k, i, j)];
   double v3 = st.w[g.c(IVZ, p, k, i, j)];
   double rho = st.w[g.c(IDN, p, k, i, j)];
   double radius = g.x1v[i], yrad = qdiv(1., radius);
   double v_2 = v2 + v3 * cosine, v_3 = v3 + v2 * cosine;
   double pr;
-  if (g.eos == EOS_SW) {
+  if (eos_of<EK>(g) == EOS_SW) {
     pr = 0.5 * rho * rho;
   } else {
     pr = st.w[g.c(IPR, p, k, i, j)];
@@ -961,7 +986,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   double src3 = -yovrC * (pr + rho * v2 * v2 * sine2) - div_rn(rho * v1 * v_3, radius, yrad);
   out[IVZ] += wght * src3;
   // sources
-  if (g.eos == EOS_IDEAL && g.grav != 0.0) {
+  if (eos_of<EK>(g) == EOS_IDEAL && g.grav != 0.0) {
     double src = wght * rho * g.grav;
     out[IVX] += src;
     out[IEN] += src * v1;
@@ -1015,14 +1040,14 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
     double dflx;
     if (!MIX) {
       dflx = a1r * st.sflx[0][g.f(n, p, k, i + 1, j)] - a1l * st.sflx[0][g.f(n, p, k, i, j)];
-      if (g.eos == EOS_SW) dflx = 0.;
+      if (eos_of<EK>(g) == EOS_SW) dflx = 0.;
       double s2r = X2R ? rf->s2r[n] : st.sflx[1][g.f(n, p, k, i, j + 1)];
       double s2l = X2R ? rf->s2l[n] : st.sflx[1][g.f(n, p, k, i, j)];
       dflx += a2r * s2r - a2l * s2l;
       dflx += a3r * st.sflx[2][g.f(n, p, k + 1, i, j)] - a3l * st.sflx[2][g.f(n, p, k, i, j)];
     } else {
       const ADDR(1) float *S0 = st.fsflx[0], *S1 = st.fsflx[1], *S2 = st.fsflx[2];
-      dflx = g.eos == EOS_SW ? 0.
+      dflx = eos_of<EK>(g) == EOS_SW ? 0.
                              : a1r * (double)S0[g.f(n, p, k, i + 1, j)] -
                                    a1l * (double)S0[g.f(n, p, k, i, j)];
       dflx += a2r * (double)(X2R ? rf->fs2r[n] : S1[g.f(n, p, k, i, j + 1)]) -
@@ -1042,7 +1067,7 @@ __device__ __forceinline__ void update_cell(const DGrid &g, const DState &st, in
   }
 }
 
-template <bool MIX, bool NOAVG = false>
+template <bool MIX, bool NOAVG = false, int EK = -1>
 __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, int lb, StageW sw) {
   size_t t = (size_t)lb * BLK + threadIdx.x;
   // the validity flag describes the primitive conversion that follows this
@@ -1059,20 +1084,22 @@ __device__ __forceinline__ void update_body(const DGrid &g, const DState &st, in
   int k = g.ks + (int)(r % N), p = (int)(r / N);
   double out[5];
   double cth;
-  update_cell<MIX, NOAVG>(g, st, p, k, i, j, sw, out, nullptr, &cth);
-  ADDR(1) double *dst = g.eos == EOS_IDEAL ? st.du : st.u;
-  for (int n = 0; n < g.nv; ++n)
-    if (!g.skipv(n)) dst[g.c(n, p, k, i, j)] = out[n];
+  update_cell<MIX, NOAVG, false, false, EK>(g, st, p, k, i, j, sw, out, nullptr, &cth);
+  ADDR(1) double *dst = eos_of<EK>(g) == EOS_IDEAL ? st.du : st.u;
+  for (int n = 0; n < nv_of<EK>(g); ++n)
+    if (!skip_of<EK>(g, n)) dst[g.c(n, p, k, i, j)] = out[n];
   // shallow water within a step: the stage's new state is final here, so its
   // primitive state is written now (k_prim_ghost then covers the ghosts)
-  if (sw.prim && g.eos == EOS_SW) prim_from(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor, cth);
+  if (sw.prim && eos_of<EK>(g) == EOS_SW)
+    prim_from<EK>(g, st, p, k, i, j, out, sw.dfloor, sw.sfloor, cth);
 }
 
-template <bool MIX, bool NOAVG = false>
+// EK: the equation of state of every instance of the launch, or -1 (any)
+template <bool MIX, bool NOAVG = false, int EK = -1>
 __global__ void __launch_bounds__(BLK) k_update(const Inst *__restrict__ I_, Work w, StageW sw) {
   WORK_LOOP(w) {
     WORK_INST(w);
-    update_body<MIX, NOAVG>(g, st, lb, sw);
+    update_body<MIX, NOAVG, EK>(g, st, lb, sw);
   }
 }
 
@@ -1207,7 +1234,7 @@ __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &
       for (int n = 0; n < RF_NT; ++n) tr[n] = __shfl_down(tl[n], 1);
     }
   } else {
-    if (act) face_flux64<1>(g, st, p, kk, fa, i, k, j, fl);
+    if (act) face_flux64<1, EOS_IDEAL>(g, st, p, kk, fa, i, k, j, fl);
     if (SCAL && act) {
 #pragma unroll
       for (int n = 0; n < RF_NT; ++n) {
@@ -1231,7 +1258,7 @@ __device__ __forceinline__ void face2_update_body(const DGrid &g, const DState &
   if (cell) {
     double out[5];
     RegFlux rf{fl, fr, f1l, f1r, sl, sr, gl, gr, tl, tr};
-    update_cell<MIX, true, !SCAL, true>(g, st, p, k, i, j, sw, out, &rf);
+    update_cell<MIX, true, !SCAL, true, EOS_IDEAL>(g, st, p, k, i, j, sw, out, &rf);
 #pragma unroll
     for (int n = 0; n < 5; ++n) st.du[g.c(n, p, k, i, j)] = out[n];
   }
@@ -2661,6 +2688,7 @@ struct Launcher {
   int cap[K_NKIND] = {};    // workgroups of a persistent launch (0: one per block)
   int shape[K_NKIND] = {};  // tuned block shape of a kind (0: the default): the x1
                             // kinds' waves per 64-column block (2, 4 or 8)
+  int eos = -1;             // equation of state of every instance (-1: they differ)
   std::vector<Inst> h_inst;     // host copies of the instances (the tuner reads them)
   size_t lds_x1 = 0;        // dynamic LDS of the x1 kernels (largest instance)
   // live per-kernel timing with HIP events on the stream
@@ -2813,6 +2841,9 @@ void launcher_setup(Launcher &L, const std::vector<exo_handle *> &hs,
     inst[i].ned = (int)h->setup.edges.size();
     L.lds_x1 = std::max(L.lds_x1, (size_t)(h->g.nc1 + 1) * 64 * sizeof(double));
   }
+  L.eos = n > 0 ? hs[0]->g.eos : -1;
+  for (int i = 1; i < n; ++i)
+    if (hs[i]->g.eos != L.eos) L.eos = -1;
   L.h_inst = inst;
   for (int kind = 0; kind < K_NKIND; ++kind) {
     long long acc = 0;
@@ -3034,10 +3065,17 @@ void init_handle(exo_handle *h) {
   launcher_setup(h->L, std::vector<exo_handle *>{h}, h->allocs);
 }
 
+// the full primitive conversion specialised for the launch's equation of state
+void launch_prim(Launcher &L, int kid) {
+  if (L.eos == EOS_SW) launch(L, K_PRIM, kid, k_prim<EOS_SW>, BLK, 0, DFLOOR, SFLOOR);
+  else if (L.eos == EOS_IDEAL) launch(L, K_PRIM, kid, k_prim<EOS_IDEAL>, BLK, 0, DFLOOR, SFLOOR);
+  else launch(L, K_PRIM, kid, k_prim<>, BLK, 0, DFLOOR, SFLOOR);
+}
+
 void launch_fill_ghosts(Launcher &L) {
   launch(L, K_GHOST_U, KID_GHOST, k_ghost, BLK, 0, 0);
   launch(L, K_GHOST_S, KID_GHOST, k_ghost, BLK, 0, 1);
-  launch(L, K_PRIM, KID_PRIM, k_prim, BLK, 0, DFLOOR, SFLOOR);
+  launch_prim(L, KID_PRIM);
   HIPCHK(hipGetLastError());
 }
 
@@ -3061,14 +3099,23 @@ void launch_x1(Launcher &L, int kind, int kid) {
 
 // fused: within a stage, the x2 faces of the instances k_face2_update covers
 // are left to it
+// the FP64 face kernel specialised for the launch's equation of state
+template <int dir>
+void launch_face64(Launcher &L, int kind, int kid) {
+  if (L.eos == EOS_SW) launch(L, kind, kid, k_flux_h<dir, EOS_SW>, BLK, 0);
+  else if (L.eos == EOS_IDEAL) launch(L, kind, kid, k_flux_h<dir, EOS_IDEAL>, BLK, 0);
+  else launch(L, kind, kid, k_flux_h<dir>, BLK, 0);
+}
+
 void launch_fluxes(Launcher &L, bool fused = false) {
   launch_x1(L, K_X1, KID_FLUX_X1);
   launch_x1(L, K_X1_MIX, KID_FLUX_X1);
   launch(L, K_EDGE, KID_EDGE, k_edge_states, BLK, 0);
   launch(L, K_EDGE_MIX, KID_EDGE, k_edge_states_mix, BLK, 0);
   launch(L, K_EFLX, KID_EDGE, k_edge_flux, BLK, 0);
-  launch(L, fused ? K_FACE_NF : K_FACE, KID_FLUX_X2, k_flux_h<1>, BLK, 0);
-  launch(L, K_FACE, KID_FLUX_X3, k_flux_h<2>, BLK, 0);
+  if (fused) launch(L, K_FACE_NF, KID_FLUX_X2, k_flux_h<1, EOS_SW>, BLK, 0);
+  else launch_face64<1>(L, K_FACE, KID_FLUX_X2);
+  launch_face64<2>(L, K_FACE, KID_FLUX_X3);
   if (!fused) launch(L, K_FACE_MIX, KID_FLUX_X2, k_flux_h_mix<1, false>, BLK, 0);
   launch(L, K_FACE_MIX, KID_FLUX_X3, k_flux_h_mix<2, false>, BLK, 0);
   launch(L, K_FACE_MIX_SW, KID_FLUX_X2, k_flux_h_mix<1, true>, BLK, 0);
@@ -3100,7 +3147,7 @@ void launch_update_implicit(Launcher &L, int stage) {
   int st0 = stage - 1;
   StageW sw{SS_DELTA[st0], SS_G1[st0], SS_G2[st0], SS_BETA[st0], st0 == 0 ? 1 : 0,
             1, DFLOOR, SFLOOR};
-  launch(L, K_UPD_RS, KID_UPDATE, k_update<false>, BLK, 0, sw);
+  launch(L, K_UPD_RS, KID_UPDATE, k_update<false, false, EOS_SW>, BLK, 0, sw);
   launch(L, K_FUSE2, KID_UPDATE, k_face2_update<false, false>, BLK, 0, sw);
   launch(L, K_FUSE2_S, KID_UPDATE, k_face2_update<true, false>, BLK, 0, sw);
   launch(L, K_FUSE2_MIX, KID_UPDATE, k_face2_update<false, true>, BLK, 0, sw);
@@ -3134,7 +3181,7 @@ void launch_dt(Launcher &L, double dt_fixed) {
 void launch_fill_ghosts_stage(Launcher &L) {
   launch(L, K_GHOST_U, KID_GHOST, k_ghost, BLK, 0, 0);
   launch(L, K_GHOST_S, KID_GHOST, k_ghost, BLK, 0, 1);
-  launch(L, K_PRIM_REST, KID_PRIM, k_prim, BLK, 0, DFLOOR, SFLOOR);
+  launch(L, K_PRIM_REST, KID_PRIM, k_prim<EOS_IDEAL>, BLK, 0, DFLOOR, SFLOOR);
   launch(L, K_PRIM_GHOST, KID_PRIM, k_prim_ghost, BLK, 0, DFLOOR, SFLOOR);
   HIPCHK(hipGetLastError());
 }
@@ -3196,15 +3243,15 @@ void tune_launcher(Launcher &L, int reps) {
           // step: its inputs come from memory just written, not from caches
           // warmed by the previous repetition (the conversion's own time is
           // the same for every candidate)
-          if (kind != K_PRIM) launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR);
+          if (kind != K_PRIM) launch_prim(L, -1);
           switch (kind) {
-            case K_PRIM: launch(L, K_PRIM, -1, k_prim, BLK, 0, DFLOOR, SFLOOR); break;
+            case K_PRIM: launch_prim(L, -1); break;
             case K_GHOST_U: launch(L, K_GHOST_U, -1, k_ghost, BLK, 0, 0); break;
             case K_X1: launch_x1(L, K_X1, -1); break;
             case K_X1_MIX: launch_x1(L, K_X1_MIX, -1); break;
             case K_EDGE: launch(L, K_EDGE, -1, k_edge_states, BLK, 0); break;
             case K_EDGE_MIX: launch(L, K_EDGE_MIX, -1, k_edge_states_mix, BLK, 0); break;
-            case K_FACE: launch(L, K_FACE, -1, k_flux_h<1>, BLK, 0); break;
+            case K_FACE: launch_face64<1>(L, K_FACE, -1); break;
             case K_FACE_MIX: launch(L, K_FACE_MIX, -1, k_flux_h_mix<1, false>, BLK, 0); break;
             case K_FACE_MIX_SW: launch(L, K_FACE_MIX_SW, -1, k_flux_h_mix<1, true>, BLK, 0); break;
             case K_DT: launch(L, K_DT, -1, k_dt, BLK, 0); break;
