@@ -4,7 +4,9 @@ totals, on the GPU.
 A batch runs every phase as one launch over all members (optionally persistent
 and replayed from a captured HIP graph); each member's result must be
 bit-identical to stepping it alone, which is itself bit-exact against the
-oracle in FP64 (test_gpu_parity.py)."""
+oracle in FP64 (test_gpu_parity.py).  The members below mix equations of state
+and Riemann solvers, so the batch runs the generic kernels while each member
+alone runs the kernels specialised for its own: the two agree bit for bit."""
 import math
 
 import numpy as np
