@@ -97,13 +97,17 @@ def step_compulsory_bytes_per_cell(nv, mixed):
 
 
 # kernel of the live profile -> its name in the rocprofv3 PMC summary, per form
+# (the trailing template arguments are the kernels' specialisations for this
+# workload: equation of state 1 = ideal gas, Riemann solver 3 = LMARS, -1 = read
+# per instance)
 PMC_NAMES = {
-    "fp64": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, false>",
-             "flux_x1": "k_flux_x1<4>", "flux_x3": "k_flux_h<2>",
+    "fp64": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, false, 3>",
+             "flux_x1": "k_flux_x1<4, -1>", "flux_x3": "k_flux_h<2, 1, 3>",
              "edge_states": "k_edge_states", "ghost": "k_ghost",
-             "prim": "k_prim", "dt": "k_dt"},
-    "mixed": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, true>",
-              "flux_x1": "k_flux_x1_mix<4>", "flux_x3": "k_flux_h_mix<2, false>", "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim",
+             "prim": "k_prim<1>", "dt": "k_dt"},
+    "mixed": {"implicit": "k_implicit<true>", "update": "k_face2_update<false, true, 3>",
+              "flux_x1": "k_flux_x1_mix<4, 3>", "flux_x3": "k_flux_h_mix<2, false, 3>",
+              "edge_states": "k_edge_states_mix", "ghost": "k_ghost", "prim": "k_prim<1>",
               "dt": "k_dt"},
 }
 # launches of each kernel per SSPRK3 step of the headline workload (a kernel
@@ -133,7 +137,7 @@ def pmc_name(kernel, form, tuned=None):
     if name and kernel == "flux_x1" and tuned:
         sh = tuned.get("x1_mix_shape" if form == "mixed" else "x1_shape")
         if sh:
-            name = name.replace("<4>", f"<{sh}>")
+            name = name.replace("<4,", f"<{sh},")
     return name
 
 
