@@ -225,10 +225,20 @@ def run_gpu(ex, precision, steps, warmup, perturb=0.0, implicit_flag=9):
     g.profile_begin()
     g.step(min(steps, 20))
     prof = g.profile_end()
+    # latency of a single step taken on its own: host wall time around one
+    # graph replay and the read-back of its dt, over 50 steps
+    lat = []
+    for _ in range(50):
+        ts = time.perf_counter()
+        g.step(1)
+        lat.append((time.perf_counter() - ts) * 1e3)
     finite = bool(np.isfinite(u).all())
     g.close()
     sec = t1 - t0
-    return dict(cells=cells, sec=sec, dt=dt, prof=prof, finite=finite, nv=5, u=u, tuned=tuned)
+    step_latency = {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                    "steps": len(lat)}
+    return dict(cells=cells, sec=sec, dt=dt, prof=prof, finite=finite, nv=5, u=u, tuned=tuned,
+                step_latency_ms=step_latency)
 
 
 def partial_correction(ex, steps, warmup):
@@ -757,10 +767,11 @@ def main():
                  "dt_last_s": r64["dt"], "kernels": kernel_rooflines(r64["prof"], r64["cells"],
                                                                    r64["nv"], False,
                                                                    implicit_flag=iflag),
-                 "tuned_launches": r64["tuned"]},
+                 "tuned_launches": r64["tuned"], "step_latency_ms": r64["step_latency_ms"]},
         "mixed": {"value": value_mx, "ms_per_step": rmx["sec"] * 1e3 / args.steps,
                   "dt_last_s": rmx["dt"], "kernels": kernels_mx,
-                  "counts_for_metric": mixed_meets, "tuned_launches": rmx["tuned"]},
+                  "counts_for_metric": mixed_meets, "tuned_launches": rmx["tuned"],
+                  "step_latency_ms": rmx["step_latency_ms"]},
         "mixed_over_fp64": value_mx / value64,
         "mixed_rel_l2_vs_fp64": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"), l2)},
         "mixed_rel_linf_vs_fp64": {n: float(x) for n, x in zip(("rho", "m1", "m2", "m3", "E"),
