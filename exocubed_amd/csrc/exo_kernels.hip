@@ -777,11 +777,13 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
 }
 
 // five wavefronts per SIMD: 96 VGPRs without spills, 4 % faster than the
-// compiler's four on the x3 faces of 6 x 96 x 96 x 64.  EK: the equation of
-// state of every instance of the launch, or -1 (any); RK: their Riemann
-// solver, or -1.
+// compiler's four on the x3 faces of 6 x 96 x 96 x 64; the shallow-water x2
+// faces at six (80 VGPRs without spills, 4 % faster on the 256-instance
+// ensemble; its x3 faces spill there).  EK: the equation of state of every
+// instance of the launch, or -1 (any); RK: their Riemann solver, or -1.
 template <int dir, int EK = -1, int RK = -1>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(BLK)
+__attribute__((amdgpu_waves_per_eu(dir == 1 && EK == EOS_SW ? 6 : 5)))
 k_flux_h(const Inst *__restrict__ I_, Work w) {
   WORK_LOOP(w) {
     WORK_INST(w);
