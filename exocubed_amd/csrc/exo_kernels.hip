@@ -4250,7 +4250,7 @@ int exo_write_netcdf_fields(const exo_config *cfg, const char *basename, const c
 // Host-only introspection (no device needed): the set-up tables a handle
 // would upload.  which: 0 ghost descriptors, 1 edge descriptors, 2 metric
 // tables [T_NTAB][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f of the six panels,
-// 5 x3f of the six panels.
+// 5 x3f of the six panels, 6 per-row tangent tables [T1_NTAB][6][nc2 + 1].
 int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t *count) {
   if (!cfg || !count) return EXO_ERR_ARG;
   Setup s;
@@ -4275,6 +4275,7 @@ int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap, size_t
         tmp.insert(tmp.end(), v.begin(), v.end());
       }
       src = tmp.data(); n = tmp.size(); esz = sizeof(double); break;
+    case 6: src = s.tab1.data(); n = s.tab1.size(); esz = sizeof(double); break;
     default: return EXO_ERR_ARG;
   }
   *count = n;

@@ -295,7 +295,9 @@ EXO_API int exo_math_batch(exo_handle *h, int op, const double *x, double y, dou
 
 /* Host-only introspection of the set-up tables (no GPU needed): which 0 ghost
  * remap descriptors, 1 edge-exchange descriptors, 2 metric tables
- * [20][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f and 5 x3f of the six panels.
+ * [20][6][nc3][nc2], 3 x1f|x1v|dx1f, 4 x2f and 5 x3f of the six panels,
+ * 6 per-row tangent tables [8][6][nc2 + 1] (x = tan x2v, tan x2f, C =
+ * sqrt(1 + x^2) at centres and faces, then the same for y along x3).
  * With buf == NULL only *count is returned. */
 EXO_API int exo_describe(const exo_config *cfg, int which, void *buf, size_t cap_bytes,
                  size_t *count);

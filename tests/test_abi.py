@@ -82,6 +82,30 @@ def test_host_only_tables(ex):
     assert len(pairs) == len(e)
 
 
+def test_row_tables_give_the_cell_tables_bits(ex):
+    """The one-layer shallow-water update forms tan and cos theta of a cell
+    from the per-row tangent tables (x, y, C = sqrt(1 + x^2), D = sqrt(1 +
+    y^2)) instead of reading the per-cell tables: with IEEE operations in the
+    set-up expression's order the two must agree bit for bit."""
+    for cfg in (sw_config(ex, nx=8), atm_config(ex, nx=12, nx1=4)):
+        nc = cfg.nx + 6
+        t1 = ex.describe(cfg, 6).reshape(8, 6, nc + 1)
+        t2 = ex.describe(cfg, 2).reshape(20, 6, nc, nc)
+        x, c = t1[0][:, None, :nc], t1[2][:, None, :nc]  # [p][1][j]
+        y, d = t1[4][:, :nc, None], t1[6][:, :nc, None]  # [p][k][1]
+        cosc = (-x * y) / (c * d)
+        assert np.array_equal(cosc.view(np.uint64), t2[0].view(np.uint64))
+        assert np.array_equal(np.broadcast_to(x, t2[13].shape), t2[13])  # tan x2v
+        assert np.array_equal(np.broadcast_to(y, t2[14].shape), t2[14])  # tan x3v
+        # C and D are the maps' sqrt(X X + 1) and sqrt(Y Y + 1) (Coriolis);
+        # the centre rows have nc entries, the face rows nc + 1
+        assert np.array_equal(np.sqrt(t1[0, :, :nc] * t1[0, :, :nc] + 1), t1[2, :, :nc])
+        assert np.array_equal(np.sqrt(t1[4, :, :nc] * t1[4, :, :nc] + 1), t1[6, :, :nc])
+        # the face rows too, which the set-up tables are formed from
+        assert np.array_equal(np.sqrt(1.0 + t1[1] * t1[1]), t1[3])
+        assert np.array_equal(np.sqrt(1.0 + t1[5] * t1[5]), t1[7])
+
+
 def test_invalid_configs_are_rejected_without_a_gpu(ex):
     L = ex.lib()
     n = C.c_size_t()
