@@ -585,17 +585,27 @@ __device__ __forceinline__ void flux_x1_body(const DGrid &g, const DState &st, i
       load(5, fi + 3);
     }
   }
-  // passive scalars: WENO of r along x1, upwinded by the mass flux
-  for (int n = 0; n < g.nt; ++n)
+  // passive scalars: WENO of r along x1, upwinded by the mass flux; a window
+  // of 6 levels (fi - 3 .. fi + 2) slides up the column, the next level and
+  // mass flux loaded before the face's arithmetic
+  for (int n = 0; n < g.nt; ++n) {
+    double r[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) r[s] = st.r[g.c(n, p, k, f0 - 3 + s, j)];
+    double m = f0 < f1 ? st.flx[0][g.f(IDN, p, k, f0, j)] : 0.;
     for (int fi = f0; fi < f1; ++fi) {
-      double r0 = st.r[g.c(n, p, k, fi - 3, j)], r1 = st.r[g.c(n, p, k, fi - 2, j)];
-      double r2 = st.r[g.c(n, p, k, fi - 1, j)], r3 = st.r[g.c(n, p, k, fi, j)];
-      double r4 = st.r[g.c(n, p, k, fi + 1, j)], r5 = st.r[g.c(n, p, k, fi + 2, j)];
-      double rl = weno5(r4, r3, r2, r1, r0);
-      double rr = weno5(r1, r2, r3, r4, r5);
-      double m = st.flx[0][g.f(IDN, p, k, fi, j)];
+      bool more = fi + 1 < f1;
+      double rn = more ? st.r[g.c(n, p, k, fi + 3, j)] : 0.;
+      double mn = more ? st.flx[0][g.f(IDN, p, k, fi + 1, j)] : 0.;
+      double rl = weno5(r[4], r[3], r[2], r[1], r[0]);
+      double rr = weno5(r[1], r[2], r[3], r[4], r[5]);
       st.sflx[0][g.f(n, p, k, fi, j)] = m >= 0.0 ? m * rl : m * rr;
+#pragma unroll
+      for (int s = 0; s < 5; ++s) r[s] = r[s + 1];
+      r[5] = rn;
+      m = mn;
     }
+  }
 #undef PSF
 }
 
@@ -2307,6 +2317,8 @@ __device__ __forceinline__ void flux_x1_mix_body(const DGrid &g, const DState &s
       q[5][IPR] = (float)(nxt[IPR] - psv);
     }
   }
+  // (a window sliding up the column, as in the FP64 sweep, measured 3-4 %
+  // slower here: 0.091 -> 0.094 ms with two scalars on 6 x 64 x 64 x 64)
   for (int n = 0; n < g.nt; ++n)
     for (int fi = f0; fi < f1; ++fi) {
       float r[6];
