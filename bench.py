@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
+# autotune launch widths and shapes before timing (off with --no-tune: a
+# profiler's counter passes slow kernels unevenly and would mislead the tuner,
+# so the profiled runs take the default launches)
+TUNE = True
 
 
 def log(msg):
@@ -211,7 +215,7 @@ def run_gpu(ex, precision, steps, warmup, perturb=0.0, implicit_flag=9):
     g.upload(u0, s0)
     # launch widths and block shapes from the autotuner (timed before the
     # warm-up steps; the first steps are then taken from the uploaded state)
-    tuned = {k: v for k, v in g.tune(reps=3).items() if v}
+    tuned = {k: v for k, v in g.tune(reps=3).items() if v} if TUNE else {}
     g.upload(u0, s0)
     if warmup:
         g.step(warmup)
@@ -334,7 +338,7 @@ def ensemble(ex, precision, steps, n_inst=256, keep=16):
         g.sync()
     t_alone = time.perf_counter() - t0
     b = ex.Batch(members, graph=True)
-    caps = b.tune(reps=2)
+    caps = b.tune(reps=2) if TUNE else {}
     b.step(2)  # warm-up: graph capture
     m0 = b.sums()[:, 0]
     t0 = time.perf_counter()
@@ -583,10 +587,14 @@ def main():
     ap.add_argument("--implicit-flag", type=int, default=9, choices=(1, 9),
                     help="column correction of the headline workload: 9 full 5x5 (the "
                          "benchmark), 1 partial 3x3 (for profiling k_implicit3)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch widths and shapes, no autotuning (for counter passes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc pass)")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_tune
     if args.copy_calibration:
         import torch
         a = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
