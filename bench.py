@@ -579,6 +579,8 @@ def main():
     ap.add_argument("--serving-sweep", action="store_true",
                     help="only the serving stream, both forms, at max_active 4, 8, 16 and 32")
     ap.add_argument("--only-headline", action="store_true")
+    ap.add_argument("--only-large", action="store_true",
+                    help="only the 6 x 384 x 384 x 128 run (--large-steps steps, for profiling)")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="only the 256-instance shallow-water batch, both forms (for profiling)")
     ap.add_argument("--only-serving", action="store_true",
@@ -649,6 +651,9 @@ def main():
         vmx, _ = serving(ex, ex.PREC_MIXED, args.requests, keep=0)
         print(json.dumps({"serving": {"fp64": v64, "mixed": vmx,
                                       "mixed_over_fp64": vmx["value"] / v64["value"]}}))
+        return
+    if args.only_large:
+        print(json.dumps({"large": large(ex, args.large_steps)}))
         return
     if args.only_ensemble:
         e64, _ = ensemble(ex, ex.PREC_FP64, args.ensemble_steps)
