@@ -749,7 +749,7 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
   size_t total = (size_t)6 * N * ni * na;
   if (t >= total) return;
   // lanes walk j in both directions: x2 faces (j fastest over N+1 faces),
-  // x3 faces (j fastest over N cells, face index k slower)
+  // x3 faces (j fastest over N cells, then the face index k, then i)
   int kk, fa, i, p;
   if (dir == 1) {
     fa = (int)(t % na);
@@ -759,12 +759,14 @@ __device__ __forceinline__ void flux_h_body(const DGrid &g, const DState &st, in
     kk = (int)(r % N);
     p = (int)(r / N);
   } else {
+    // j fastest, then the face index k, then i: the six-row stencil along k
+    // of consecutive faces is re-read within a few blocks, from L2
     kk = (int)(t % N);
     size_t r = t / N;
-    i = g.is + (int)(r % ni);
-    r /= ni;
     fa = (int)(r % na);
-    p = (int)(r / na);
+    r /= na;
+    i = g.is + (int)(r % ni);
+    p = (int)(r / ni);
   }
   int k, j;
   if (dir == 1) { k = g.ks + kk; j = g.js + fa; }
@@ -2439,12 +2441,14 @@ __device__ __forceinline__ void flux_h_mix_body(const DGrid &g, const DState &st
     kk = (int)(r % N);
     p = (int)(r / N);
   } else {
+    // j fastest, then the face index k, then i: the six-row stencil along k
+    // of consecutive faces is re-read within a few blocks, from L2
     kk = (int)(t % N);
     size_t r = t / N;
-    i = g.is + (int)(r % ni);
-    r /= ni;
     fa = (int)(r % na);
-    p = (int)(r / na);
+    r /= na;
+    i = g.is + (int)(r % ni);
+    p = (int)(r / ni);
   }
   int k = dir == 1 ? g.ks + kk : g.ks + fa;
   int j = dir == 1 ? g.js + fa : g.js + kk;
